@@ -1,0 +1,291 @@
+/*
+ * sg_oracle.c -- CPU restatement of the SplineGrids.jl grid-evaluation path.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP path and the CPU baseline
+ * timed by bench.py.  The product never links or calls this file.
+ *
+ * Built with -ffp-contract=off so that a*b+c is two roundings, as in the
+ * reference's Julia kernels (Julia does not contract without @fastmath or
+ * muladd).  Each function cites the reference file:line it restates.
+ */
+#include "sg_oracle.h"
+
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+int32_t sgo_max_threads(void) {
+#ifdef _OPENMP
+  return (int32_t)omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+/* src/util_kernels.jl:1-20 (launched from src/knot_vector.jl:31-36). */
+int32_t sgo_expand_knots(const double* values, const int32_t* mults, int32_t k,
+                         double* knots_all) {
+  int32_t pos = 0;
+  for (int32_t i = 0; i < k; ++i)
+    for (int32_t r = 0; r < mults[i]; ++r) knots_all[pos++] = values[i];
+  return pos;
+}
+
+/* src/util_kernels.jl:22-49. */
+void sgo_sample_indices(const double* t, int32_t m, const double* knots_all,
+                        int32_t n_knots, int32_t p, int32_t* idx1) {
+  for (int32_t l = 0; l < m; ++l) {
+    const double s = t[l];
+    int32_t count = 0;
+    for (int32_t k = 0; k < n_knots; ++k) {
+      if (s < knots_all[k]) break; /* :306-311, ties go right */
+      ++count;
+    }
+    const int32_t lo = p + 1, hi = n_knots - p - 1; /* :314-318 */
+    if (count < lo) count = lo;
+    if (count > hi) count = hi;
+    idx1[l] = count;
+  }
+}
+
+/* src/spline_dimension.jl:160-215.  Same loop nest and operation order;
+ * 1-based reference indices are shifted to 0-based here.  eval/eval_prev are
+ * per-sample scratch of shape [p+1, D+1]. */
+void sgo_basis(const double* knots_all, const double* t, const int32_t* idx1,
+               int32_t m, int32_t p, int32_t max_deriv, double* eval) {
+  const int32_t w = p + 1, nd = max_deriv + 1;
+  double* cur = (double*)calloc((size_t)w * nd, sizeof(double));
+  double* prev = (double*)calloc((size_t)w * nd, sizeof(double));
+#define CUR(r, o) cur[(r) + w * (o)]
+#define PREV(r, o) prev[(r) + w * (o)]
+  for (int32_t l = 0; l < m; ++l) {
+    const double tt = t[l];
+    const int32_t i = idx1[l]; /* 1-based span */
+    memset(prev, 0, sizeof(double) * (size_t)w * nd); /* :174-176 */
+    memset(cur, 0, sizeof(double) * (size_t)w * nd);
+    CUR(0, 0) = 1.0; /* :179 */
+    PREV(0, 0) = 1.0; /* :180 */
+    for (int32_t k = 1; k <= p; ++k) { /* :183 */
+      memset(cur, 0, sizeof(double) * (size_t)w * nd); /* :185-187 */
+      for (int32_t kk = 1; kk <= k; ++kk) { /* :189 */
+        /* knots_all[i + kk - k], knots_all[i + kk] are 1-based (:190-191) */
+        const double t_min = knots_all[i + kk - k - 1];
+        const double t_max = knots_all[i + kk - 1];
+        const double dt = t_max - t_min;
+        double b_prev = PREV(kk - 1, 0);
+        const double frac = b_prev / dt;
+        CUR(kk - 1, 0) += frac * (t_max - tt); /* :196 */
+        CUR(kk, 0) = frac * (tt - t_min);      /* :197 */
+        for (int32_t d = 1; d <= max_deriv + k - p; ++d) { /* :200 */
+          b_prev = PREV(kk - 1, d - 1);
+          const double contrib = b_prev * k / dt; /* :202 */
+          CUR(kk - 1, d) -= contrib;              /* :203 */
+          CUR(kk, d) = contrib;                   /* :204 */
+        }
+      }
+      if (k != p) memcpy(prev, cur, sizeof(double) * (size_t)w * nd); /* :209-213 */
+    }
+    for (int32_t o = 0; o < nd; ++o)
+      for (int32_t r = 0; r < w; ++r)
+        eval[l + (size_t)m * (r + (size_t)w * o)] = CUR(r, o);
+  }
+#undef CUR
+#undef PREV
+  free(cur);
+  free(prev);
+}
+
+/* Shared geometry of one grid: strides and per-dimension offsets. */
+typedef struct {
+  int32_t nin, nout;
+  int64_t n_cp, n_samples;
+  int64_t cp_stride[SGO_MAX_NIN], s_stride[SGO_MAX_NIN];
+  int32_t width[SGO_MAX_NIN];
+  int64_t n_terms;
+} geom_t;
+
+static void geom_init(geom_t* g, int32_t nin, int32_t nout, const int32_t* n,
+                      const int32_t* m, const int32_t* p) {
+  g->nin = nin;
+  g->nout = nout;
+  g->n_cp = 1;
+  g->n_samples = 1;
+  g->n_terms = 1;
+  for (int32_t d = 0; d < nin; ++d) {
+    g->cp_stride[d] = g->n_cp;
+    g->s_stride[d] = g->n_samples;
+    g->n_cp *= n[d];
+    g->n_samples *= m[d];
+    g->width[d] = p[d] + 1;
+    g->n_terms *= p[d] + 1;
+  }
+}
+
+/* src/spline_grid.jl:129-183, templated on the value type T.  One sample:
+ * J (multi-index) -> out[c * ostride] for c < nout. */
+#define DEFINE_EVAL_ONE(T, SUF)                                                \
+  static void eval_one_##SUF(const geom_t* g, const int32_t* m,               \
+                             const int32_t* p, const T* const* tables,        \
+                             const int32_t* const* idx1,                      \
+                             const int32_t* deriv, const T* P, const T* W,    \
+                             const int32_t* J, T* out, int64_t ostride) {     \
+    const int32_t nin = g->nin, nout = g->nout;                               \
+    int64_t base[SGO_MAX_NIN];                                                \
+    const T* tab[SGO_MAX_NIN];                                                \
+    for (int32_t d = 0; d < nin; ++d) {                                       \
+      base[d] = (int64_t)idx1[d][J[d]] - p[d] - 1; /* :145-146, 0-based */    \
+      tab[d] = tables[d] + J[d] + (int64_t)m[d] * ((int64_t)(p[d] + 1) *      \
+                                                   deriv[d]);                 \
+    }                                                                         \
+    T acc[64];                                                                \
+    for (int32_t c = 0; c < nout; ++c) acc[c] = (T)0; /* :138-140 */          \
+    T denom = (T)0;                                                           \
+    int32_t I[SGO_MAX_NIN];                                                   \
+    for (int32_t d = 0; d < nin; ++d) I[d] = 0;                               \
+    for (int64_t term = 0; term < g->n_terms; ++term) { /* :148 */            \
+      int64_t cp = 0;                                                         \
+      T prod = (T)1; /* :153 */                                               \
+      for (int32_t d = 0; d < nin; ++d) {                                     \
+        cp += (base[d] + I[d]) * g->cp_stride[d];                             \
+        prod *= tab[d][(int64_t)m[d] * I[d]]; /* :155-158 */                  \
+      }                                                                       \
+      if (W) { /* :161-165 */                                                 \
+        prod = prod * W[cp];                                                  \
+        denom += prod;                                                        \
+      }                                                                       \
+      for (int32_t c = 0; c < nout; ++c) /* :171-174 */                       \
+        acc[c] += prod * P[cp + g->n_cp * c];                                 \
+      for (int32_t d = 0; d < nin; ++d) { /* column-major odometer */         \
+        if (++I[d] < g->width[d]) break;                                      \
+        I[d] = 0;                                                             \
+      }                                                                       \
+    }                                                                         \
+    if (W)                                                                    \
+      for (int32_t c = 0; c < nout; ++c) acc[c] /= denom; /* :178-182 */      \
+    for (int32_t c = 0; c < nout; ++c) out[c * ostride] = acc[c];             \
+  }
+
+DEFINE_EVAL_ONE(double, f64)
+DEFINE_EVAL_ONE(float, f32)
+
+#define DEFINE_EVAL_DIRECT(T, SUF)                                             \
+  void sgo_eval_direct_##SUF(                                                 \
+      int32_t nin, int32_t nout, const int32_t* n, const int32_t* m,          \
+      const int32_t* p, const int32_t* D, const T* const* tables,             \
+      const int32_t* const* idx1, const int32_t* deriv, const T* P,           \
+      const T* W, T* eval, int32_t last_begin, int32_t last_end,              \
+      int32_t nthreads) {                                                     \
+    (void)D;                                                                  \
+    geom_t g;                                                                 \
+    geom_init(&g, nin, nout, n, m, p);                                        \
+    const int64_t slab = g.s_stride[nin - 1]; /* samples per last index */    \
+    const int64_t n_out = slab * (int64_t)(last_end - last_begin);            \
+    if (nthreads < 1) nthreads = 1;                                           \
+    _Pragma("omp parallel for schedule(static) num_threads(nthreads)")        \
+    for (int64_t q = 0; q < n_out; ++q) {                                     \
+      int32_t J[SGO_MAX_NIN];                                                 \
+      int64_t r = q;                                                          \
+      for (int32_t d = 0; d < nin - 1; ++d) {                                 \
+        J[d] = (int32_t)(r % m[d]);                                           \
+        r /= m[d];                                                            \
+      }                                                                       \
+      J[nin - 1] = (int32_t)r + last_begin;                                   \
+      eval_one_##SUF(&g, m, p, tables, idx1, deriv, P, W, J, eval + q,        \
+                     n_out);                                                  \
+    }                                                                         \
+  }
+
+DEFINE_EVAL_DIRECT(double, f64)
+DEFINE_EVAL_DIRECT(float, f32)
+
+void sgo_eval_points_f64(int32_t nin, int32_t nout, const int32_t* n,
+                         const int32_t* m, const int32_t* p, const int32_t* D,
+                         const double* const* tables,
+                         const int32_t* const* idx1, const int32_t* deriv,
+                         const double* P, const double* W,
+                         const int64_t* lin, int64_t npts, double* out,
+                         int32_t nthreads) {
+  (void)D;
+  geom_t g;
+  geom_init(&g, nin, nout, n, m, p);
+  if (nthreads < 1) nthreads = 1;
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+  for (int64_t q = 0; q < npts; ++q) {
+    int32_t J[SGO_MAX_NIN];
+    int64_t r = lin[q];
+    for (int32_t d = 0; d < nin; ++d) {
+      J[d] = (int32_t)(r % m[d]);
+      r /= m[d];
+    }
+    eval_one_f64(&g, m, p, tables, idx1, deriv, P, W, J, out + q, npts);
+  }
+}
+
+/* src/adjoint.jl:1-40 (kernel) and :52-83 (zeroing at :61). */
+#define DEFINE_ADJOINT(T, SUF, ATOMIC)                                         \
+  static void adjoint_##SUF(int32_t nin, int32_t nout, const int32_t* n,      \
+                            const int32_t* m, const int32_t* p,               \
+                            const T* const* tables,                           \
+                            const int32_t* const* idx1, const int32_t* deriv, \
+                            const T* ebar, T* Pbar, int32_t nthreads) {       \
+    geom_t g;                                                                 \
+    geom_init(&g, nin, nout, n, m, p);                                        \
+    memset(Pbar, 0, sizeof(T) * (size_t)(g.n_cp * nout));                     \
+    if (nthreads < 1) nthreads = 1;                                           \
+    _Pragma("omp parallel for schedule(static) num_threads(nthreads)")        \
+    for (int64_t q = 0; q < g.n_samples; ++q) {                               \
+      int32_t J[SGO_MAX_NIN], I[SGO_MAX_NIN];                                 \
+      int64_t base[SGO_MAX_NIN];                                              \
+      const T* tab[SGO_MAX_NIN];                                              \
+      int64_t r = q;                                                          \
+      for (int32_t d = 0; d < nin; ++d) {                                     \
+        J[d] = (int32_t)(r % m[d]);                                           \
+        r /= m[d];                                                            \
+        base[d] = (int64_t)idx1[d][J[d]] - p[d] - 1; /* :248-249 */           \
+        tab[d] = tables[d] + J[d] +                                           \
+                 (int64_t)m[d] * ((int64_t)(p[d] + 1) * deriv[d]);            \
+        I[d] = 0;                                                             \
+      }                                                                       \
+      for (int64_t term = 0; term < g.n_terms; ++term) { /* :251 */           \
+        int64_t cp = 0;                                                       \
+        T prod = (T)1;                                                        \
+        for (int32_t d = 0; d < nin; ++d) {                                   \
+          cp += (base[d] + I[d]) * g.cp_stride[d];                            \
+          prod *= tab[d][(int64_t)m[d] * I[d]]; /* :258-261 */                \
+        }                                                                     \
+        for (int32_t c = 0; c < nout; ++c) { /* :264-268 */                   \
+          const T v = prod * ebar[q + g.n_samples * c];                       \
+          ATOMIC                                                              \
+          Pbar[cp + g.n_cp * c] += v;                                         \
+        }                                                                     \
+        for (int32_t d = 0; d < nin; ++d) {                                   \
+          if (++I[d] < g.width[d]) break;                                     \
+          I[d] = 0;                                                           \
+        }                                                                     \
+      }                                                                       \
+    }                                                                         \
+  }
+
+DEFINE_ADJOINT(double, f64, )
+DEFINE_ADJOINT(float, f32, _Pragma("omp atomic"))
+
+void sgo_adjoint_direct_f64(int32_t nin, int32_t nout, const int32_t* n,
+                            const int32_t* m, const int32_t* p,
+                            const int32_t* D, const double* const* tables,
+                            const int32_t* const* idx1, const int32_t* deriv,
+                            const double* ebar, double* Pbar) {
+  (void)D;
+  /* one thread: a fixed summation order, so the checker is deterministic */
+  adjoint_f64(nin, nout, n, m, p, tables, idx1, deriv, ebar, Pbar, 1);
+}
+
+void sgo_adjoint_direct_f32(int32_t nin, int32_t nout, const int32_t* n,
+                            const int32_t* m, const int32_t* p,
+                            const int32_t* D, const float* const* tables,
+                            const int32_t* const* idx1, const int32_t* deriv,
+                            const float* ebar, float* Pbar, int32_t nthreads) {
+  (void)D;
+  adjoint_f32(nin, nout, n, m, p, tables, idx1, deriv, ebar, Pbar, nthreads);
+}

@@ -1,0 +1,100 @@
+/*
+ * sg_oracle.h -- CPU restatement of the SplineGrids.jl grid-evaluation path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP path is
+ * compared against (and the CPU baseline that bench.py times).  Nothing in the
+ * product (splinegrids.jl_amd/) links, loads or calls it.
+ *
+ * Conventions follow the reference exactly:
+ *   - arrays are column-major (first index fastest), the output channel is the
+ *     slowest axis of control points and eval;
+ *   - sample_indices are 1-based spans, as produced by
+ *     src/util_kernels.jl:22-49;
+ *   - basis tables are laid out [m, p+1, D+1] (src/spline_dimension.jl:135-141).
+ *
+ * Every function is restated from the reference file:line given beside it.
+ * Pinning (what checks each function): see the header of oracle/oracle.py.
+ */
+#ifndef SG_ORACLE_H
+#define SG_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SGO_MAX_NIN 8
+
+/* src/knot_vector.jl:28-37 + src/util_kernels.jl:1-20: repeat values[i]
+ * mults[i] times.  Returns the expanded length. */
+int32_t sgo_expand_knots(const double* values, const int32_t* mults, int32_t k,
+                         double* knots_all);
+
+/* src/util_kernels.jl:22-49: i = #{knots_all[k] <= t} (linear scan, stops at
+ * the first knot > t), clamped to [p+1, n_knots-p-1]; 1-based. */
+void sgo_sample_indices(const double* t, int32_t m, const double* knots_all,
+                        int32_t n_knots, int32_t p, int32_t* idx1);
+
+/* src/spline_dimension.jl:160-215: Cox-de Boor triangle plus derivative
+ * recursion, same operation order, in double.  eval is [m, p+1, D+1]. */
+void sgo_basis(const double* knots_all, const double* t, const int32_t* idx1,
+               int32_t m, int32_t p, int32_t max_deriv, double* eval);
+
+/* src/spline_grid.jl:119-183: direct per-sample sum over (p+1)^Nin control
+ * points, NURBS weights fused, eval overwritten.  tables[d] is the [m_d, p_d+1,
+ * D_d+1] table of dimension d, idx1[d] its 1-based spans.  W may be NULL.
+ * Only the slab j_last in [last_begin, last_end) of the slowest sample
+ * dimension is evaluated and written (eval is then sized for that slab);
+ * pass 0, m[nin-1] for the whole grid.  nthreads > 1 uses OpenMP. */
+void sgo_eval_direct_f64(int32_t nin, int32_t nout, const int32_t* n,
+                         const int32_t* m, const int32_t* p, const int32_t* D,
+                         const double* const* tables,
+                         const int32_t* const* idx1, const int32_t* deriv,
+                         const double* P, const double* W, double* eval,
+                         int32_t last_begin, int32_t last_end,
+                         int32_t nthreads);
+
+/* Same algorithm with Tv = Float32 (the reference's default value type):
+ * float tables, control points, weights and arithmetic. */
+void sgo_eval_direct_f32(int32_t nin, int32_t nout, const int32_t* n,
+                         const int32_t* m, const int32_t* p, const int32_t* D,
+                         const float* const* tables,
+                         const int32_t* const* idx1, const int32_t* deriv,
+                         const float* P, const float* W, float* eval,
+                         int32_t last_begin, int32_t last_end,
+                         int32_t nthreads);
+
+/* The direct sum of src/spline_grid.jl:148-182 at a list of samples given by
+ * their column-major linear sample index; out is [npts, nout] column-major. */
+void sgo_eval_points_f64(int32_t nin, int32_t nout, const int32_t* n,
+                         const int32_t* m, const int32_t* p, const int32_t* D,
+                         const double* const* tables,
+                         const int32_t* const* idx1, const int32_t* deriv,
+                         const double* P, const double* W,
+                         const int64_t* lin, int64_t npts, double* out,
+                         int32_t nthreads);
+
+/* src/adjoint.jl:1-40 and :52-83: Pbar is zeroed (:61), then every sample
+ * scatters prod(B) * ebar into the (p+1)^Nin control points it touches.
+ * Sequential, so the summation order is fixed. */
+void sgo_adjoint_direct_f64(int32_t nin, int32_t nout, const int32_t* n,
+                            const int32_t* m, const int32_t* p,
+                            const int32_t* D, const double* const* tables,
+                            const int32_t* const* idx1, const int32_t* deriv,
+                            const double* ebar, double* Pbar);
+
+/* The same scatter in Float32 with OpenMP atomics, as the reference's
+ * threaded CPU backend runs it (Atomix.@atomic, src/adjoint.jl:35-37). */
+void sgo_adjoint_direct_f32(int32_t nin, int32_t nout, const int32_t* n,
+                            const int32_t* m, const int32_t* p,
+                            const int32_t* D, const float* const* tables,
+                            const int32_t* const* idx1, const int32_t* deriv,
+                            const float* ebar, float* Pbar, int32_t nthreads);
+
+int32_t sgo_max_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
