@@ -1,0 +1,320 @@
+"""Benchmark of the grid-evaluation hot path on one MI355X.
+
+Default workload (one "step" = one pass of the hot path over one batch):
+the 2-D -> 1 freeform surface, 128 x 128 control points, degree (3, 3),
+4096 x 4096 samples, non-uniform clamped knots with 10 % doubled interior
+knots, control points N(0, 1) in FP32, derivative fields (0,0), (1,0),
+(0,1), (1,1) evaluated in one fused launch.  value = sample-points/s =
+fields * samples / time.  Inputs are resident in HBM before timing starts.
+
+Prints one JSON line.  Other workloads (--workload c1|c3|c3n|c4|c4a) are
+for profiling and parity runs.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PEAK_HBM_GBS = 8000.0
+METRIC = "sample-pts/s + HBM GB/s %peak; split-FP16 MFMA vs FP32 at ≤1e-6 rel err"
+SEED0 = 0x5EED0000
+
+
+# --------------------------------------------------------------------------
+# workloads (shared with tests/test_gpu_configs.py)
+# --------------------------------------------------------------------------
+
+def c2_knot_values(rng, n=128, p=3, n_doubled=12):
+    """Clamped knots: end multiplicity p+1, sorted U[0,1) interior values of
+    which n_doubled have multiplicity 2 (132 knots for n = 128, p = 3)."""
+    n_inner = n - p - 1 - n_doubled  # interior slots minus the extra copies
+    while True:
+        inner = np.sort(rng.uniform(0.0, 1.0, n_inner)).astype(np.float32)
+        vals = np.concatenate([[0.0], inner, [1.0]]).astype(np.float32)
+        if len(np.unique(vals)) == len(vals):
+            break
+    mults = np.ones(len(vals), dtype=np.int32)
+    mults[0] = mults[-1] = p + 1
+    mults[1 + rng.choice(n_inner, n_doubled, replace=False)] = 2
+    assert mults.sum() == n + p + 1
+    return vals, mults
+
+
+def workload(name: str):
+    """Description of a named workload: dims, arrays, fields, dtype."""
+    if name == "c1":  # README shape, FP64, no derivatives
+        rng = np.random.default_rng(SEED0 + 1)
+        return dict(name="c1: 3-D->4 B-spline, n=(10,10,5), p=(2,3,2), m=(50,50,25), FP64",
+                    n=(10, 10, 5), p=(2, 3, 2), m=(50, 50, 25), D=(0, 0, 0), nout=4,
+                    dtype=np.float64, knots=None, fields=[(0, 0, 0)],
+                    P=rng.uniform(0, 1, (10, 10, 5, 4)), W=None)
+    if name == "c2":
+        rng = np.random.default_rng(SEED0 + 2)
+        knots = [c2_knot_values(rng), c2_knot_values(rng)]
+        P = rng.standard_normal((128, 128, 1)).astype(np.float32)
+        return dict(name="c2: 2-D->1 freeform surface, n=(128,128), p=(3,3), m=(4096,4096), "
+                         "4 derivative fields, FP32",
+                    n=(128, 128), p=(3, 3), m=(4096, 4096), D=(1, 1), nout=1,
+                    dtype=np.float32, knots=knots,
+                    fields=[(0, 0), (1, 0), (0, 1), (1, 1)], P=P, W=None)
+    if name in ("c3", "c3n"):
+        rng = np.random.default_rng(SEED0 + 3)
+        P = rng.uniform(-1, 1, (64, 64, 32, 3)).astype(np.float32)
+        W = rng.uniform(0.5, 2.0, (64, 64, 32)).astype(np.float32)
+        return dict(name="c3: 3-D->3 NURBS, n=(64,64,32), p=(3,3,2), m=(512,512,256), FP32",
+                    n=(64, 64, 32), p=(3, 3, 2), m=(512, 512, 256), D=(0, 0, 0), nout=3,
+                    dtype=np.float32, knots=None, fields=[(0, 0, 0)], P=P,
+                    W=W if name == "c3" else None)
+    if name == "copy":  # calibration: a plain device copy of 1 GiB
+        return dict(name="copy: 1 GiB device-to-device copy (counter calibration)",
+                    n=(1,), p=(0,), m=(1 << 28,), D=(0,), nout=1, dtype=np.float32,
+                    knots=None, fields=[(0,)], P=None, W=None, copy=True)
+    if name in ("c4", "c4a"):
+        rng = np.random.default_rng(SEED0 + 4)
+        P = rng.uniform(-1, 1, (40, 40, 20, 3)).astype(np.float32)
+        return dict(name="c4: 3-D->3 B-spline, n=(40,40,20), p=3, m=(400,400,200), FP32"
+                         + (", adjoint" if name == "c4a" else ""),
+                    n=(40, 40, 20), p=(3, 3, 3), m=(400, 400, 200), D=(0, 0, 0), nout=3,
+                    dtype=np.float32, knots=None, fields=[(0, 0, 0)], P=P, W=None,
+                    adjoint=(name == "c4a"))
+    raise ValueError(f"unknown workload {name}")
+
+
+def build_grid(sg, wl, device="cuda"):
+    import torch
+
+    dims = []
+    for i in range(len(wl["n"])):
+        kw = {}
+        if wl["knots"] is not None:
+            vals, mults = wl["knots"][i]
+            kw["knot_vector"] = sg.KnotVector(vals.astype(wl["dtype"]), mults)
+        dims.append(sg.SplineDimension(wl["n"][i], wl["p"][i], wl["m"][i],
+                                       max_derivative_order=wl["D"][i],
+                                       float_type=wl["dtype"], device=device, **kw))
+    grid = (sg.NURBSGrid if wl["W"] is not None else sg.SplineGrid)(tuple(dims), wl["nout"])
+    grid.control_points.copy_(wl["P"].astype(wl["dtype"]))
+    if wl["W"] is not None:
+        grid.weights.copy_(sg.from_numpy_colmajor(wl["W"], grid.dtype, device))
+    torch.cuda.synchronize()
+    return grid
+
+
+def algorithmic_bytes(wl, adjoint=False) -> int:
+    """Bytes one pass must move at minimum: every output written once, every
+    input read once, per-dimension spans and value-type tables read once."""
+    s = np.dtype(wl["dtype"]).itemsize
+    nsamp, ncp = math.prod(wl["m"]), math.prod(wl["n"])
+    F, nout = len(wl["fields"]), wl["nout"]
+    tables = sum(m * (p + 1) * (D + 1) * s + 4 * m
+                 for m, p, D in zip(wl["m"], wl["p"], wl["D"]))
+    if adjoint:
+        return nsamp * nout * s + ncp * nout * s + tables
+    b = F * nsamp * nout * s + ncp * nout * s + tables
+    if wl["W"] is not None:
+        b += ncp * s
+    return b
+
+
+# --------------------------------------------------------------------------
+# CPU baseline: the oracle's restatement of the reference algorithm
+# --------------------------------------------------------------------------
+
+def cpu_baseline(wl, budget_s: float = 10.0):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    nthreads = max(1, min(16, os.cpu_count() or 1))
+    dims = []
+    for i in range(len(wl["n"])):
+        if wl["knots"] is not None:
+            vals, mults = wl["knots"][i]
+            knots = O.expand_knots(vals.astype(np.float64), mults)
+        else:
+            v, mu = O.default_knot_values(wl["n"][i], wl["p"][i], dtype=wl["dtype"])
+            knots = O.expand_knots(v.astype(np.float64), mu)
+        t = O.sample_points((knots[0], knots[-1]), wl["m"][i], wl["dtype"]).astype(np.float64)
+        dims.append(O.Dim(knots, t, wl["p"][i], wl["D"][i]))
+    P = wl["P"].astype(wl["dtype"])
+    W = None if wl["W"] is None else wl["W"].astype(wl["dtype"])
+    slab_rows = max(1, min(wl["m"][-1], 512 * 4096 // max(1, math.prod(wl["m"][:-1]))))
+    done, t0 = 0, time.perf_counter()
+    start = 0
+    while time.perf_counter() - t0 < budget_s:
+        stop = min(start + slab_rows, wl["m"][-1])
+        for f in wl["fields"]:
+            O.eval_direct(dims, P, deriv=f, W=W, dtype=wl["dtype"],
+                          last_range=(start, stop), nthreads=nthreads)
+            done += math.prod(wl["m"][:-1]) * (stop - start)
+        start = 0 if stop >= wl["m"][-1] else stop
+    dt = time.perf_counter() - t0
+    return {"value": done / dt, "unit": "sample-pts/s", "cores": nthreads, "kind": "port",
+            "sample": f"reference algorithm (direct (p+1)^Nin sum per sample, "
+                      f"{np.dtype(wl['dtype']).name}) over slabs of {slab_rows} rows of the "
+                      f"slowest sample axis, all fields, {done} sample-points in {dt:.1f} s"}
+
+
+def sampled_parity(sg, grid, wl, outs, n_points: int = 20000):
+    """Max |GPU - FP64 oracle| over random samples of each field, relative to
+    max|field| on the GPU (the oracle consumes the same Tv-rounded inputs)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    dims = [O.Dim(s.host_knots(), s.host_sample_points(), s.degree, s.max_derivative_order)
+            for s in grid.spline_dimensions]
+    rng = np.random.default_rng(1)
+    nsamp = math.prod(wl["m"])
+    lin = rng.integers(0, nsamp, n_points)
+    P = wl["P"].astype(wl["dtype"]).astype(np.float64)
+    W = None if wl["W"] is None else wl["W"].astype(wl["dtype"]).astype(np.float64)
+    errs = []
+    for f, out in zip(wl["fields"], outs):
+        want = O.eval_points(dims, P, lin, deriv=f, W=W, nthreads=max(1, min(16, os.cpu_count() or 1)))
+        flat = sg.vec_colmajor(out).view(-1, 1) if wl["nout"] == 1 else \
+            out.permute(*range(out.ndim - 1, -1, -1)).reshape(wl["nout"], -1).t()
+        got = flat[lin].double().cpu().numpy()
+        scale = float(out.abs().max())
+        errs.append(float(np.abs(got - want).max()) / scale)
+    return errs
+
+
+def traffic_from_profiles(workload_name: str):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if any."""
+    path = os.path.join(ROOT, "profiles", f"{workload_name}_pmc.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    return d.get("traffic_bytes_per_launch")
+
+
+# --------------------------------------------------------------------------
+# main
+# --------------------------------------------------------------------------
+
+def copy_calibration(args, wl):
+    """Time torch's device copy of 1 GiB; its counters calibrate FETCH/WRITE_SIZE."""
+    import torch
+
+    src = torch.ones(wl["m"][0], dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(args.warmup):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(args.steps):
+        dst.copy_(src)
+    stop.record()
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(stop) / args.steps
+    nbytes = 2 * src.numel() * 4
+    print(json.dumps({"metric": "copy GB/s", "value": nbytes / (ms * 1e-3) / 1e9,
+                      "unit": "GB/s", "ms_per_step": ms, "bytes_per_step": nbytes,
+                      "config": {"workload": wl["name"]}}), flush=True)
+    return 0
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="c2")
+    ap.add_argument("--tier", default="native")
+    ap.add_argument("--cpu-budget", type=float, default=10.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--separate-fields", action="store_true",
+                    help="one launch per field instead of the fused launch")
+    args = ap.parse_args(argv)
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on one GPU only: the evaluation shards by sample slabs with no "
+              "communication, so a multi-GPU figure is N independent single-GPU runs "
+              "(see DESIGN.md); use --gpus 1.", file=sys.stderr)
+        return 2
+
+    import torch
+
+    import __graft_entry__
+
+    sg = __graft_entry__.package()
+    wl = workload(args.workload)
+    if wl.get("copy"):
+        return copy_calibration(args, wl)
+    grid = build_grid(sg, wl)
+    adjoint = wl.get("adjoint", False)
+    fields = wl["fields"]
+    outs = [grid.eval] + [torch.empty_like(grid.eval) for _ in fields[1:]]
+    if adjoint:
+        ebar = sg.colmajor(torch.randn(grid.eval.shape, dtype=grid.dtype, device="cuda"))
+        pbar = sg.colmajor_zeros(grid.control_points.shape, grid.dtype, "cuda")
+
+    def step():
+        if adjoint:
+            sg.evaluate_adjoint_(grid, eval=ebar, control_points=pbar, tier=args.tier)
+        elif len(fields) == 1:
+            sg.evaluate_(grid, derivative_order=fields[0], tier=args.tier)
+        elif args.separate_fields:
+            for f, o in zip(fields, outs):
+                sg.evaluate_(grid, derivative_order=f, eval=o, tier=args.tier)
+        else:
+            sg.evaluate_fields_(grid, fields, outs, tier=args.tier)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t_wall = time.perf_counter()
+    start.record(stream)
+    for _ in range(args.steps):
+        step()
+    stop.record(stream)
+    torch.cuda.synchronize()
+    t_wall = time.perf_counter() - t_wall
+    ms = start.elapsed_time(stop) / args.steps
+    nsamp = math.prod(wl["m"])
+    n_fields = 1 if adjoint else len(fields)
+    value = n_fields * nsamp / (ms * 1e-3)
+    B = algorithmic_bytes(wl, adjoint)
+    achieved = B / (ms * 1e-3) / 1e9
+    launches = 1 if (adjoint or len(fields) == 1 or not args.separate_fields) else len(fields)
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "sample-pts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if wl["dtype"] == np.float64 else "f32",
+        "data": "synthetic (seeded inputs of the workload's shape)",
+        "config": {"workload": wl["name"], "tier": args.tier, "fields": len(fields),
+                   "fused_fields": not args.separate_fields, "launches_per_step": launches},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS,
+                     "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
+                     "traffic": traffic_from_profiles(args.workload),
+                     "algorithmic_bytes_per_step": B},
+        "wall_s_timed": t_wall,
+    }
+    if not adjoint:
+        out["max_rel_err_vs_fp64"] = sampled_parity(sg, grid, wl, outs)
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(wl, args.cpu_budget)
+        out["cpu_baseline"]["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
