@@ -72,10 +72,11 @@ def workload(name: str):
                     n=(64, 64, 32), p=(3, 3, 2), m=(512, 512, 256), D=(0, 0, 0), nout=3,
                     dtype=np.float32, knots=None, fields=[(0, 0, 0)], P=P,
                     W=W if name == "c3" else None)
-    if name == "copy":  # calibration: a plain device copy of 1 GiB
-        return dict(name="copy: 1 GiB device-to-device copy (counter calibration)",
+    if name in ("copy", "fill"):  # calibration: plain device copy / fill of 1 GiB
+        what = "device-to-device copy" if name == "copy" else "fill (write only)"
+        return dict(name=f"{name}: 1 GiB {what} (bandwidth and counter calibration)",
                     n=(1,), p=(0,), m=(1 << 28,), D=(0,), nout=1, dtype=np.float32,
-                    knots=None, fields=[(0,)], P=None, W=None, copy=True)
+                    knots=None, fields=[(0,)], P=None, W=None, copy=name)
     if name in ("c4", "c4a"):
         rng = np.random.default_rng(SEED0 + 4)
         P = rng.uniform(-1, 1, (40, 40, 20, 3)).astype(np.float32)
@@ -205,18 +206,20 @@ def copy_calibration(args, wl):
 
     src = torch.ones(wl["m"][0], dtype=torch.float32, device="cuda")
     dst = torch.empty_like(src)
+    fill = wl["copy"] == "fill"
+    op = (lambda: dst.fill_(1.5)) if fill else (lambda: dst.copy_(src))
     for _ in range(args.warmup):
-        dst.copy_(src)
+        op()
     torch.cuda.synchronize()
     start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     start.record()
     for _ in range(args.steps):
-        dst.copy_(src)
+        op()
     stop.record()
     torch.cuda.synchronize()
     ms = start.elapsed_time(stop) / args.steps
-    nbytes = 2 * src.numel() * 4
-    print(json.dumps({"metric": "copy GB/s", "value": nbytes / (ms * 1e-3) / 1e9,
+    nbytes = (1 if fill else 2) * src.numel() * 4
+    print(json.dumps({"metric": f"{wl['copy']} GB/s", "value": nbytes / (ms * 1e-3) / 1e9,
                       "unit": "GB/s", "ms_per_step": ms, "bytes_per_step": nbytes,
                       "config": {"workload": wl["name"]}}), flush=True)
     return 0

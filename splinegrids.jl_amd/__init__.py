@@ -13,6 +13,8 @@ from .control_points import DefaultControlPoints, get_n_control_points, obtain
 from .spline_grid import (NURBSGrid, Plan, SplineGrid, evaluate_, evaluate_adjoint_,
                           evaluate_fields_, reshape_colmajor, vec_colmajor)
 from .validation import validate_partial_derivatives, validate_spline_grid
+from .autograd import SplineEval, spline_eval
+from .linear_map import LinearMap
 from .utils import colmajor, colmajor_empty, colmajor_zeros, from_numpy_colmajor, to_numpy_colmajor
 
 __version__ = "0.1.0"
@@ -23,5 +25,5 @@ __all__ = [
     "SplineGrid", "evaluate_", "evaluate_adjoint_", "evaluate_fields_",
     "reshape_colmajor", "vec_colmajor", "validate_partial_derivatives",
     "validate_spline_grid", "colmajor", "colmajor_empty", "colmajor_zeros",
-    "from_numpy_colmajor", "to_numpy_colmajor",
+    "from_numpy_colmajor", "to_numpy_colmajor", "SplineEval", "spline_eval", "LinearMap",
 ]

@@ -6,19 +6,22 @@
 // thread, sums run in a fixed order, and results are bit-reproducible.
 //
 // Streaming path (Nin = 2, 3; monotone spans; p1, p2 <= 5), two kernels:
-//   adj_stream  one workgroup per (64-sample tile of dim 1, chunk of rows of
-//               dim 2, sample j3).  Each lane (sample j1) walks its wave's
-//               rows j2 and accumulates U[k2] += B2[j2, b] * ebar[j1, j2, j3]
-//               in a sliding window of p2+1 registers; an entry that leaves
-//               the window is final and goes to the wave's LDS slab.  The
-//               workgroup then contracts the slabs along dimension 1 (each
-//               (k1, k2) output sums the lanes whose band holds k1) into a
-//               partial slab of the tile's band window, written to the plan's
-//               workspace.
+//   adj_stream  one workgroup per (64-sample tile of dimension 1, chunk of
+//               rows of dimension 2, range of samples j3 of dimension 3).
+//               For each j3 of its range:
+//     A  each lane (sample j1) walks its wave's rows j2 and accumulates
+//        U[k2] += B2[j2, b] * ebar[j1, j2, j3] in a sliding window of p2+1
+//        registers; an entry that leaves the window is final and goes to
+//        the wave's LDS slab.  Loads are issued four rows ahead.
+//     B  the workgroup contracts the slabs along dimension 1: output
+//        (k1, k2) sums the lanes whose band holds k1 (B1 transposed per k1
+//        in LDS, 16 bytes per LDS read).
+//     C  the (k1, k2) slab is added, weighted by B3[j3, a], into a ring of
+//        p3+1 k3-slabs in LDS; a k3 that leaves the band of the next j3 is
+//        final for this workgroup and is written to its partial window.
 //   adj_gather  one thread per control point (k1, k2, k3, c): sums the
-//               partial slabs of the tiles and chunks whose windows hold
-//               (k1, k2), weighted by B3[j3, k3 - s3 + p3] over the samples j3
-//               whose band holds k3 (the transposed band list of dimension 3).
+//               partial windows of the few workgroups whose windows hold
+//               (k1, k2, k3).
 // General path (any Nin, any degree, any sample order): adj_csr, one thread
 // per control point, walking each dimension's transposed band list.
 #include "sg_internal.hpp"
@@ -33,151 +36,226 @@ struct AdjStreamArgs {
   T* part;
   const T* tab1;  // [m1][p1+1] at the requested order
   const T* tab2;  // [m2][P2+1]
+  const T* tab3;  // [m3][p3+1] (3-D) or null (2-D)
   const int32_t* span1;
   const int32_t* span2;
-  int m1, m2, p1;
+  const int32_t* span3;
+  int m1, m2, m3, p1, p3;
   int c0, nout;
-  int rows_per_wave;
-  int w1max, w2max, w2wave;
-  int n_tiles1, n_chunks2;
+  int rows_per_wave, j3_per_block;
+  int ntx, nty, xcd_map;
+  int w1max, w2max, w2wave, w3max;
   long long m12, nsamp;
 };
+
+// sum_{l in [l0, l1)} a[l] * b[l] for 16-byte aligned rows, 16 bytes at a time
+__device__ inline float dot_lanes(const float* a, const float* b, int l0, int l1) {
+  float s = 0.f;
+  for (int l = l0; l < l1; l += 4) {
+    const float4 x = *reinterpret_cast<const float4*>(a + l);
+    const float4 y = *reinterpret_cast<const float4*>(b + l);
+    s += x.x * y.x;
+    s += x.y * y.y;
+    s += x.z * y.z;
+    s += x.w * y.w;
+  }
+  return s;
+}
+__device__ inline double dot_lanes(const double* a, const double* b, int l0, int l1) {
+  double s = 0.0;
+  for (int l = l0; l < l1; l += 2) {
+    const double2 x = *reinterpret_cast<const double2*>(a + l);
+    const double2 y = *reinterpret_cast<const double2*>(b + l);
+    s += x.x * y.x;
+    s += x.y * y.y;
+  }
+  return s;
+}
+
+constexpr int kRowBatch = 4;  // rows whose loads are in flight together
 
 template <typename T, int P2, int NC>
 __global__ __launch_bounds__(kThreads) void adj_stream(AdjStreamArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem[];
-  constexpr int PM = kMaxStreamDegree;
-  T* S = reinterpret_cast<T*>(smem);                        // [4][w2wave][NC][64]
-  T* Bl = S + static_cast<size_t>(kWaves) * a.w2wave * NC * kLanes;  // [64][PM+1]
-  int* Sl = reinterpret_cast<int*>(Bl + kLanes * (PM + 1));  // [64]
-  int* wlo = Sl + kLanes;                                    // [4]
-  int* whi = wlo + kWaves;                                   // [4]
-  int* llo = whi + kWaves;                                   // [w1max]
-  int* lhi = llo + a.w1max;                                  // [w1max]
+  // S   [4][w2wave][NC][64]  finished k2 entries of each wave (one j3)
+  // BlT [w1max][64]          B1 of lane l at k1 = lo1 + q1 (0 off the band)
+  // R3  [p3+1][w1max*w2max*NC] ring of k3-slabs
+  T* S = reinterpret_cast<T*>(smem);
+  T* BlT = S + static_cast<size_t>(kWaves) * a.w2wave * NC * kLanes;
+  T* R3 = BlT + static_cast<size_t>(a.w1max) * kLanes;
+  const int slab = a.w1max * a.w2max * NC;
+  int* wlo = reinterpret_cast<int*>(R3 + static_cast<size_t>(a.p3 + 1) * slab);
+  int* whi = wlo + kWaves;
+  int* llo = whi + kWaves;   // [w1max]
+  int* lhi = llo + a.w1max;  // [w1max]
 
   const int lane = threadIdx.x & (kLanes - 1);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kLanes);
-  const int j1_0 = blockIdx.x * kLanes;
+  const int L = xcd_logical_block(a.xcd_map, a.ntx);
+  const int bx = L % a.ntx, by = (L / a.ntx) % a.nty, bz = L / (a.ntx * a.nty);
+  const int j1_0 = bx * kLanes;
   const int chunk = a.rows_per_wave * kWaves;
-  const int r0 = blockIdx.y * chunk;
+  const int r0 = by * chunk;
   const int r1 = min(r0 + chunk, a.m2);
-  const int j3 = blockIdx.z;
+  const int j3a = bz * a.j3_per_block;
+  const int j3b = min(j3a + a.j3_per_block, a.m3);
   const int j1_last = min(j1_0 + kLanes - 1, a.m1 - 1);
-  const int p1 = a.p1;
+  const int p1 = a.p1, p3 = a.p3;
   const int lo1 = a.span1[j1_0] - p1;
   const int W1 = a.span1[j1_last] - lo1 + 1;
   const int lo2 = a.span2[r0] - P2;
   const int W2 = a.span2[r1 - 1] - lo2 + 1;
+  const int lo3 = a.tab3 ? a.span3[j3a] - p3 : 0;
+  const int nouts = W1 * W2 * NC;
 
   const bool active = (j1_0 + lane) < a.m1;
   const int j1 = active ? j1_0 + lane : a.m1 - 1;
-  const int s1 = a.span1[j1];
 
-  const int slab_elems = kWaves * a.w2wave * NC * kLanes;
-  for (int i = threadIdx.x; i < slab_elems; i += kThreads) S[i] = T(0);
-  if (wave == 0) {
-    Sl[lane] = s1;
-#pragma unroll
-    for (int t = 0; t <= PM; ++t)
-      Bl[lane * (PM + 1) + t] =
-          (active && t <= p1) ? a.tab1[static_cast<size_t>(j1) * (p1 + 1) + min(t, p1)] : T(0);
-  }
-  __syncthreads();
-
-  // ---- stage A: stream the wave's rows, sliding window over k2 ----
-  const int rb = r0 + wave * a.rows_per_wave;
-  const int re = min(rb + a.rows_per_wave, r1);
-  if (rb < re) {
-    T* Sw = S + static_cast<size_t>(wave) * a.w2wave * NC * kLanes;
-    int cur = a.span2[rb];
-    const int w0 = cur - P2;
-    T U[P2 + 1][NC];
-#pragma unroll
-    for (int q = 0; q <= P2; ++q)
-#pragma unroll
-      for (int c = 0; c < NC; ++c) U[q][c] = T(0);
-    const long long ebase = static_cast<long long>(j1) + a.m12 * j3 + a.nsamp * a.c0;
-    for (int j2 = rb; j2 < re; ++j2) {
-      const int s2 = a.span2[j2];
-      while (cur < s2) {
-        const int row = cur - P2 - w0;
-#pragma unroll
-        for (int c = 0; c < NC; ++c) Sw[(row * NC + c) * kLanes + lane] = U[0][c];
-#pragma unroll
-        for (int q = 0; q < P2; ++q)
-#pragma unroll
-          for (int c = 0; c < NC; ++c) U[q][c] = U[q + 1][c];
-#pragma unroll
-        for (int c = 0; c < NC; ++c) U[P2][c] = T(0);
-        ++cur;
-      }
-      const T* b2p = a.tab2 + static_cast<size_t>(j2) * (P2 + 1);
-      T bb[P2 + 1];
-#pragma unroll
-      for (int q = 0; q <= P2; ++q) bb[q] = b2p[q];
-      const long long eoff = ebase + static_cast<long long>(a.m1) * j2;
-#pragma unroll
-      for (int c = 0; c < NC; ++c) {
-        const T e = active ? a.E[eoff + a.nsamp * c] : T(0);
-#pragma unroll
-        for (int q = 0; q <= P2; ++q) U[q][c] += bb[q] * e;
-      }
+  for (int i = threadIdx.x; i < W1 * kLanes; i += kThreads) {
+    const int q1 = i / kLanes, l = i % kLanes;
+    const int jl = j1_0 + l;
+    T v = T(0);
+    if (jl < a.m1) {
+      const int off = lo1 + q1 - (a.span1[jl] - p1);  // slot of k1 in lane l's band
+      if (off >= 0 && off <= p1) v = a.tab1[static_cast<size_t>(jl) * (p1 + 1) + off];
     }
-#pragma unroll
-    for (int q = 0; q <= P2; ++q) {
-      const int row = cur - P2 + q - w0;
-#pragma unroll
-      for (int c = 0; c < NC; ++c) Sw[(row * NC + c) * kLanes + lane] = U[q][c];
-    }
-    if (lane == 0) {
-      wlo[wave] = w0;
-      whi[wave] = cur;
-    }
-  } else if (lane == 0) {
-    wlo[wave] = 1;
-    whi[wave] = 0;  // empty window
+    BlT[i] = v;
   }
   // lanes whose band holds k1: s1 in [k1, k1 + p1] (spans monotone in lanes)
   for (int q1 = threadIdx.x; q1 < W1; q1 += kThreads) {
     const int k1 = lo1 + q1;
-    int lo = 0, hi = kLanes;  // first lane with Sl >= k1
+    int lo = 0, hi = kLanes;  // first lane with s1 >= k1
     while (lo < hi) {
       const int mid = (lo + hi) >> 1;
       if (a.span1[min(j1_0 + mid, a.m1 - 1)] < k1) lo = mid + 1; else hi = mid;
     }
-    int lo2_ = lo, hi2_ = kLanes;  // first lane with Sl > k1 + p1
-    while (lo2_ < hi2_) {
-      const int mid = (lo2_ + hi2_) >> 1;
-      if (a.span1[min(j1_0 + mid, a.m1 - 1)] <= k1 + p1) lo2_ = mid + 1; else hi2_ = mid;
+    int lo_b = lo, hi_b = kLanes;  // first lane with s1 > k1 + p1
+    while (lo_b < hi_b) {
+      const int mid = (lo_b + hi_b) >> 1;
+      if (a.span1[min(j1_0 + mid, a.m1 - 1)] <= k1 + p1) lo_b = mid + 1; else hi_b = mid;
     }
-    llo[q1] = lo;
-    lhi[q1] = lo2_;  // exclusive
+    llo[q1] = lo & ~3;  // whole 16-byte groups; BlT is 0 off the band
+    lhi[q1] = (lo_b + 3) & ~3;
   }
-  __syncthreads();
+  for (int i = threadIdx.x; i < (p3 + 1) * slab; i += kThreads) R3[i] = T(0);
 
-  // ---- stage B: contract along dimension 1 into the partial slab ----
-  const size_t unit = (static_cast<size_t>(j3) * a.n_chunks2 + blockIdx.y) * a.n_tiles1 +
-                      blockIdx.x;
+  // partial window of this workgroup: [w3max][nout][w2max][w1max]
   const size_t plane = static_cast<size_t>(a.w1max) * a.w2max;
-  T* out = a.part + (unit * a.nout + a.c0) * plane;
-  const int nouts = W1 * W2 * NC;
-  for (int idx = threadIdx.x; idx < nouts; idx += kThreads) {
-    const int q1 = idx % W1;
-    const int rest = idx / W1;
-    const int q2 = rest % W2;
-    const int c = rest / W2;
-    const int k1 = lo1 + q1, k2 = lo2 + q2;
-    T sum = T(0);
-#pragma unroll
-    for (int w = 0; w < kWaves; ++w) {
-      if (k2 < wlo[w] || k2 > whi[w]) continue;
-      const T* Sw = S + (static_cast<size_t>(w) * a.w2wave + (k2 - wlo[w])) * NC * kLanes +
-                    c * kLanes;
-      for (int l = llo[q1]; l < lhi[q1]; ++l)
-        sum += Bl[l * (PM + 1) + (k1 - Sl[l] + p1)] * Sw[l];
+  T* part = a.part + static_cast<size_t>(L) * a.w3max * a.nout * plane;
+  // k3's ring slab is final: write it and clear it (each thread touches only
+  // the ring entries it accumulates, so no barrier is needed around this)
+  auto flush = [&](int k3) {
+    T* ring = R3 + static_cast<size_t>((k3 - lo3) % (p3 + 1)) * slab;
+    T* dst = part + static_cast<size_t>(k3 - lo3) * a.nout * plane;
+    for (int idx = threadIdx.x; idx < nouts; idx += kThreads) {
+      const int q1 = idx % W1, rest = idx / W1, q2 = rest % W2, c = rest / W2;
+      dst[(a.c0 + c) * plane + static_cast<size_t>(q2) * a.w1max + q1] = ring[idx];
+      ring[idx] = T(0);
     }
-    out[c * plane + static_cast<size_t>(q2) * a.w1max + q1] = sum;
+  };
+
+  const int rb = r0 + wave * a.rows_per_wave;
+  const int re = min(rb + a.rows_per_wave, r1);
+  T* Sw = S + static_cast<size_t>(wave) * a.w2wave * NC * kLanes;
+  int cur3 = a.tab3 ? a.span3[j3a] : 0;
+  for (int j3 = j3a; j3 < j3b; ++j3) {
+    const int s3 = a.tab3 ? a.span3[j3] : 0;
+    while (cur3 < s3) {
+      flush(cur3 - p3);
+      ++cur3;
+    }
+    __syncthreads();  // BlT/llo ready (first j3); S free again (later j3)
+
+    // ---- A: stream the wave's rows, sliding window over k2 ----
+    if (rb < re) {
+      int cur = a.span2[rb];
+      const int w0 = cur - P2;
+      T U[P2 + 1][NC];
+#pragma unroll
+      for (int q = 0; q <= P2; ++q)
+#pragma unroll
+        for (int c = 0; c < NC; ++c) U[q][c] = T(0);
+      const long long ebase = static_cast<long long>(j1) + a.m12 * j3 + a.nsamp * a.c0;
+      for (int jb = rb; jb < re; jb += kRowBatch) {
+        T e[kRowBatch][NC];
+#pragma unroll
+        for (int r = 0; r < kRowBatch; ++r) {
+          const bool ok = active && (jb + r < re);
+          const long long eoff = ebase + static_cast<long long>(a.m1) * min(jb + r, re - 1);
+#pragma unroll
+          for (int c = 0; c < NC; ++c) e[r][c] = ok ? a.E[eoff + a.nsamp * c] : T(0);
+        }
+#pragma unroll
+        for (int r = 0; r < kRowBatch; ++r) {
+          const int j2 = jb + r;
+          if (j2 >= re) break;
+          const int s2 = a.span2[j2];
+          while (cur < s2) {
+            const int row = cur - P2 - w0;
+#pragma unroll
+            for (int c = 0; c < NC; ++c) Sw[(row * NC + c) * kLanes + lane] = U[0][c];
+#pragma unroll
+            for (int q = 0; q < P2; ++q)
+#pragma unroll
+              for (int c = 0; c < NC; ++c) U[q][c] = U[q + 1][c];
+#pragma unroll
+            for (int c = 0; c < NC; ++c) U[P2][c] = T(0);
+            ++cur;
+          }
+          const T* b2p = a.tab2 + static_cast<size_t>(j2) * (P2 + 1);
+          T bb[P2 + 1];
+#pragma unroll
+          for (int q = 0; q <= P2; ++q) bb[q] = b2p[q];
+#pragma unroll
+          for (int c = 0; c < NC; ++c)
+#pragma unroll
+            for (int q = 0; q <= P2; ++q) U[q][c] += bb[q] * e[r][c];
+        }
+      }
+#pragma unroll
+      for (int q = 0; q <= P2; ++q) {
+        const int row = cur - P2 + q - w0;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) Sw[(row * NC + c) * kLanes + lane] = U[q][c];
+      }
+      if (lane == 0) {
+        wlo[wave] = w0;
+        whi[wave] = cur;
+      }
+    } else if (lane == 0) {
+      wlo[wave] = 1;
+      whi[wave] = 0;  // empty window
+    }
+    __syncthreads();
+
+    // ---- B + C: contract along dimension 1, add into the k3 ring ----
+    for (int idx = threadIdx.x; idx < nouts; idx += kThreads) {
+      const int q1 = idx % W1;
+      const int rest = idx / W1;
+      const int q2 = rest % W2;
+      const int c = rest / W2;
+      const int k2 = lo2 + q2;
+      const T* bt = BlT + q1 * kLanes;
+      const int l0 = llo[q1], l1 = lhi[q1];
+      T sum = T(0);
+#pragma unroll
+      for (int w = 0; w < kWaves; ++w) {
+        if (k2 < wlo[w] || k2 > whi[w]) continue;
+        const T* sw = S + ((static_cast<size_t>(w) * a.w2wave + (k2 - wlo[w])) * NC + c) * kLanes;
+        sum += dot_lanes(bt, sw, l0, l1);
+      }
+      if (a.tab3) {
+        const T* b3 = a.tab3 + static_cast<size_t>(j3) * (p3 + 1);
+        for (int t = 0; t <= p3; ++t) {
+          const int k3 = s3 - p3 + t;
+          R3[static_cast<size_t>((k3 - lo3) % (p3 + 1)) * slab + idx] += b3[t] * sum;
+        }
+      } else {
+        R3[idx] += sum;
+      }
+    }
   }
+  for (int k3 = cur3 - p3; k3 <= cur3; ++k3) flush(k3);
 }
 
 template <typename T>
@@ -186,14 +264,13 @@ struct AdjGatherArgs {
   T* out;
   const int32_t* tile_of_k1;
   const int32_t* chunk_of_k2;
-  const int32_t* csr3_ptr;  // dimension 3 transposed band list (3-D)
-  const int32_t* csr3_js;
-  const T* tab3;
+  const int32_t* z_of_k3;
   const int32_t* span1;
   const int32_t* span2;
+  const int32_t* span3;
   int n1, n2, nin;
   int p1, p2, p3;
-  int nout, w1max, w2max, n_tiles1, n_chunks2, rows_chunk;
+  int nout, w1max, w2max, w3max, ntx, nty, rows_chunk, j3_per_block;
   int accumulate;
   long long ncp;
 };
@@ -210,32 +287,24 @@ __global__ __launch_bounds__(256) void adj_gather(AdjGatherArgs<T> a) {
   const int k3 = static_cast<int>(r / a.n2);
   const int ta = a.tile_of_k1[2 * k1], tb = a.tile_of_k1[2 * k1 + 1];
   const int ca = a.chunk_of_k2[2 * k2], cb = a.chunk_of_k2[2 * k2 + 1];
-  const size_t plane = static_cast<size_t>(a.w1max) * a.w2max;
-  int e0 = 0, e1 = 1;
+  int za = 0, zb = 0;
   if (a.nin == 3) {
-    e0 = a.csr3_ptr[k3];
-    e1 = a.csr3_ptr[k3 + 1];
+    za = a.z_of_k3[2 * k3];
+    zb = a.z_of_k3[2 * k3 + 1];
   }
+  const size_t plane = static_cast<size_t>(a.w1max) * a.w2max;
   T sum = T(0);
-  for (int e = e0; e < e1; ++e) {
-    int j3 = 0;
-    T b3 = T(1);
-    if (a.nin == 3) {
-      const int js = a.csr3_js[e];
-      j3 = js / (a.p3 + 1);
-      b3 = a.tab3[js];
-    }
-    T inner = T(0);
+  for (int bz = za; bz <= zb; ++bz) {
+    const int lo3 = a.nin == 3 ? a.span3[bz * a.j3_per_block] - a.p3 : 0;
     for (int by = ca; by <= cb; ++by) {
       const int lo2 = a.span2[by * a.rows_chunk] - a.p2;
       for (int bx = ta; bx <= tb; ++bx) {
         const int lo1 = a.span1[bx * kLanes] - a.p1;
-        const size_t unit = (static_cast<size_t>(j3) * a.n_chunks2 + by) * a.n_tiles1 + bx;
-        inner += a.part[(unit * a.nout + c) * plane +
-                        static_cast<size_t>(k2 - lo2) * a.w1max + (k1 - lo1)];
+        const size_t unit = (static_cast<size_t>(bz) * a.nty + by) * a.ntx + bx;
+        sum += a.part[((unit * a.w3max + (k3 - lo3)) * a.nout + c) * plane +
+                      static_cast<size_t>(k2 - lo2) * a.w1max + (k1 - lo1)];
       }
     }
-    sum += b3 * inner;
   }
   a.out[idx] = a.accumulate ? a.out[idx] + sum : sum;
 }
@@ -342,17 +411,23 @@ AdjKernel<T> pick_adj(int p2, int nc) {
   return nullptr;
 }
 
-template <typename T>
-size_t stream_lds(const Plan& pl, int nc) {
-  return static_cast<size_t>(kWaves) * pl.w2wave_max * nc * kLanes * sizeof(T) +
-         kLanes * (kMaxStreamDegree + 1) * sizeof(T) +
-         (kLanes + 2 * kWaves + 2 * pl.w1max) * sizeof(int);
+}  // namespace
+
+size_t adjoint_stream_lds(const Plan& pl, int nc, size_t elem) {
+  const Tiling& t = pl.adj;
+  const int p3 = pl.nin == 3 ? pl.dims[2].p : 0;
+  return static_cast<size_t>(kWaves) * t.w2wave_max * nc * kLanes * elem +
+         static_cast<size_t>(t.w1max) * kLanes * elem +
+         static_cast<size_t>(p3 + 1) * t.w1max * t.w2max * nc * elem +
+         (2 * kWaves + 2 * t.w1max) * sizeof(int);
 }
+
+namespace {
 
 template <typename T>
 int max_nc(const Plan& pl) {
   for (int nc = 4; nc >= 1; --nc)
-    if (stream_lds<T>(pl, nc) <= static_cast<size_t>(kMaxLds)) return nc;
+    if (adjoint_stream_lds(pl, nc, sizeof(T)) <= static_cast<size_t>(kMaxLds)) return nc;
   return 0;
 }
 
@@ -362,6 +437,7 @@ hipError_t adjoint_stream(const Plan& pl, const int32_t* orders, const T* E,
   const DimDev& d1 = pl.dims[0];
   const DimDev& d2 = pl.dims[1];
   const int nin = pl.nin;
+  const Tiling& tl = pl.adj;
   AdjStreamArgs<T> a{};
   a.E = E;
   a.part = static_cast<T*>(pl.work);
@@ -371,18 +447,29 @@ hipError_t adjoint_stream(const Plan& pl, const int32_t* orders, const T* E,
   a.span2 = d2.span;
   a.m1 = d1.m;
   a.m2 = d2.m;
+  a.m3 = 1;
   a.p1 = d1.p;
+  a.p3 = 0;
+  if (nin == 3) {
+    const DimDev& d3 = pl.dims[2];
+    a.tab3 = ktab<T>(d3, orders[2]);
+    a.span3 = d3.span;
+    a.m3 = d3.m;
+    a.p3 = d3.p;
+  }
   a.nout = pl.nout;
-  a.rows_per_wave = pl.rows_per_wave;
-  a.w1max = pl.w1max;
-  a.w2max = pl.w2max;
-  a.w2wave = pl.w2wave_max;
-  a.n_tiles1 = pl.n_tiles1;
-  a.n_chunks2 = pl.n_chunks2;
+  a.rows_per_wave = tl.rows_per_wave;
+  a.j3_per_block = tl.j3_per_block;
+  a.ntx = tl.n_tiles1;
+  a.nty = tl.n_chunks2;
+  a.xcd_map = tl.xcd_map;
+  a.w1max = tl.w1max;
+  a.w2max = tl.w2max;
+  a.w2wave = tl.w2wave_max;
+  a.w3max = tl.w3max;
   a.m12 = static_cast<long long>(d1.m) * d2.m;
-  a.nsamp = a.m12 * (nin == 3 ? pl.dims[2].m : 1);
-  const unsigned gz = nin == 3 ? static_cast<unsigned>(pl.dims[2].m) : 1u;
-  const dim3 grid(pl.n_tiles1, pl.n_chunks2, gz);
+  a.nsamp = a.m12 * a.m3;
+  const dim3 grid(tl.n_tiles1 * tl.n_chunks2 * tl.n_z);
   const int ncmax = max_nc<T>(pl);
   if (ncmax == 0) return hipErrorInvalidValue;
   for (int c0 = 0; c0 < pl.nout;) {
@@ -390,7 +477,7 @@ hipError_t adjoint_stream(const Plan& pl, const int32_t* orders, const T* E,
     AdjKernel<T> k = pick_adj<T>(d2.p, nc);
     if (!k) return hipErrorInvalidValue;
     a.c0 = c0;
-    hipLaunchKernelGGL(k, grid, dim3(kThreads), stream_lds<T>(pl, nc), s, a);
+    hipLaunchKernelGGL(k, grid, dim3(kThreads), adjoint_stream_lds(pl, nc, sizeof(T)), s, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     c0 += nc;
@@ -399,8 +486,9 @@ hipError_t adjoint_stream(const Plan& pl, const int32_t* orders, const T* E,
   AdjGatherArgs<T> g{};
   g.part = static_cast<const T*>(pl.work);
   g.out = Pbar;
-  g.tile_of_k1 = pl.tile_of_k1;
-  g.chunk_of_k2 = pl.chunk_of_k2;
+  g.tile_of_k1 = tl.tile_of_k1;
+  g.chunk_of_k2 = tl.chunk_of_k2;
+  g.z_of_k3 = tl.z_of_k3;
   g.span1 = d1.span;
   g.span2 = d2.span;
   g.n1 = d1.n;
@@ -409,18 +497,17 @@ hipError_t adjoint_stream(const Plan& pl, const int32_t* orders, const T* E,
   g.p1 = d1.p;
   g.p2 = d2.p;
   if (nin == 3) {
-    const DimDev& d3 = pl.dims[2];
-    g.csr3_ptr = d3.csr_ptr;
-    g.csr3_js = d3.csr_js;
-    g.tab3 = ktab<T>(d3, orders[2]);
-    g.p3 = d3.p;
+    g.span3 = pl.dims[2].span;
+    g.p3 = pl.dims[2].p;
   }
   g.nout = pl.nout;
-  g.w1max = pl.w1max;
-  g.w2max = pl.w2max;
-  g.n_tiles1 = pl.n_tiles1;
-  g.n_chunks2 = pl.n_chunks2;
-  g.rows_chunk = pl.rows_per_wave * kWaves;
+  g.w1max = tl.w1max;
+  g.w2max = tl.w2max;
+  g.w3max = tl.w3max;
+  g.ntx = tl.n_tiles1;
+  g.nty = tl.n_chunks2;
+  g.rows_chunk = tl.chunk_rows();
+  g.j3_per_block = tl.j3_per_block;
   g.accumulate = accumulate;
   g.ncp = 1;
   for (int d = 0; d < nin; ++d) g.ncp *= pl.dims[d].n;
@@ -461,10 +548,10 @@ hipError_t adjoint_csr(const Plan& pl, const int32_t* orders, const T* E,
 }  // namespace
 
 size_t adjoint_work_bytes(const Plan& pl) {
-  if (!pl.stream_ok) return 0;
-  const size_t units = static_cast<size_t>(pl.n_tiles1) * pl.n_chunks2 *
-                       (pl.nin == 3 ? pl.dims[2].m : 1);
-  return units * pl.nout * pl.w1max * pl.w2max * sizeof(double);
+  const Tiling& t = pl.adj;
+  if (t.n_tiles1 == 0) return 0;
+  const size_t units = static_cast<size_t>(t.n_tiles1) * t.n_chunks2 * t.n_z;
+  return units * t.w3max * pl.nout * t.w1max * t.w2max * sizeof(double);
 }
 
 hipError_t launch_adjoint(const Plan& pl, const int32_t* orders,
@@ -476,7 +563,7 @@ hipError_t launch_adjoint(const Plan& pl, const int32_t* orders,
     nsamp *= d.m;
   }
   if (ncp == 0) return hipSuccess;
-  const bool stream = pl.stream_ok && nsamp > 0 &&
+  const bool stream = pl.stream_ok && pl.adj.n_tiles1 > 0 && nsamp > 0 &&
                       (dtype == SG_F32 ? max_nc<float>(pl) : max_nc<double>(pl)) > 0;
   if (dtype == SG_F32) {
     auto e = static_cast<const float*>(ebar);

@@ -5,6 +5,8 @@
 // side of the boundary so its messages and log lines are unchanged.
 #include <algorithm>
 #include <climits>
+#include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <sstream>
@@ -48,38 +50,81 @@ void free_plan(Plan* pl) {
   delete pl;
 }
 
-// Band windows of the tiling: lanes of 64 samples along dimension 1, chunks
-// of kWaves * rows_per_wave rows along dimension 2.
-void compute_tiling(Plan& pl) {
+// Band windows of a tiling: tiles of kLanes samples along dimension 1,
+// chunks of kWaves * rows_per_wave rows along dimension 2.
+void compute_tiling(const Plan& pl, Tiling& t, int rows_per_wave) {
   const DimDev& d1 = pl.dims[0];
   const DimDev& d2 = pl.dims[1];
   const std::vector<int32_t>& s1 = d1.h_span;
   const std::vector<int32_t>& s2 = d2.h_span;
-  pl.n_tiles1 = (d1.m + kLanes - 1) / kLanes;
-  pl.w1max = 0;
-  for (int t = 0; t < pl.n_tiles1; ++t) {
-    const int a = t * kLanes, b = std::min(a + kLanes, d1.m) - 1;
-    pl.w1max = std::max(pl.w1max, s1[b] - s1[a] + d1.p + 1);
+  t.rows_per_wave = rows_per_wave;
+  t.n_tiles1 = (d1.m + kLanes - 1) / kLanes;
+  t.w1max = 0;
+  for (int i = 0; i < t.n_tiles1; ++i) {
+    const int a = i * kLanes, b = std::min(a + kLanes, d1.m) - 1;
+    t.w1max = std::max(t.w1max, s1[b] - s1[a] + d1.p + 1);
   }
-  const int chunk = pl.rows_per_wave * kWaves;
-  pl.n_chunks2 = (d2.m + chunk - 1) / chunk;
-  pl.w2max = 0;
-  pl.w2wave_max = 0;
-  for (int c = 0; c < pl.n_chunks2; ++c) {
+  const int chunk = t.chunk_rows();
+  t.n_chunks2 = (d2.m + chunk - 1) / chunk;
+  t.w2max = 0;
+  t.w2wave_max = 0;
+  for (int c = 0; c < t.n_chunks2; ++c) {
     const int a = c * chunk, b = std::min(a + chunk, d2.m) - 1;
-    pl.w2max = std::max(pl.w2max, s2[b] - s2[a] + d2.p + 1);
+    t.w2max = std::max(t.w2max, s2[b] - s2[a] + d2.p + 1);
     for (int w = 0; w < kWaves; ++w) {
-      const int ra = a + w * pl.rows_per_wave;
-      const int rb = std::min(ra + pl.rows_per_wave, b + 1) - 1;
+      const int ra = a + w * rows_per_wave;
+      const int rb = std::min(ra + rows_per_wave, b + 1) - 1;
       if (ra > rb) continue;
-      pl.w2wave_max = std::max(pl.w2wave_max, s2[rb] - s2[ra] + d2.p + 1);
+      t.w2wave_max = std::max(t.w2wave_max, s2[rb] - s2[ra] + d2.p + 1);
     }
   }
 }
 
 size_t forward_lds_worst(const Plan& pl) {
   const int nc = std::min(pl.nout, 4) + (pl.nurbs ? 1 : 0);
-  return static_cast<size_t>(nc) * pl.w1max * pl.w2max * sizeof(double);
+  return static_cast<size_t>(nc) * pl.fwd.w1max * pl.fwd.w2max * sizeof(double);
+}
+
+size_t adjoint_lds_min(const Plan& pl) {  // one channel, FP64
+  return adjoint_stream_lds(pl, 1, sizeof(double));
+}
+
+// Samples of dimension 3 per adjoint workgroup: enough workgroups to fill
+// the chip (about 8 per CU), as few as possible beyond that so that each
+// k3 window is shared by many j3.
+void compute_ztiling(const Plan& pl, Tiling& t, int j3_per_block) {
+  if (pl.nin != 3) {
+    t.j3_per_block = 1;
+    t.n_z = 1;
+    t.w3max = 1;
+    return;
+  }
+  const DimDev& d3 = pl.dims[2];
+  if (j3_per_block <= 0) {
+    const long long xy = static_cast<long long>(t.n_tiles1) * t.n_chunks2;
+    const long long target = 2048;
+    j3_per_block = static_cast<int>(std::max<long long>(1, (xy * d3.m) / target));
+  }
+  t.j3_per_block = std::min(j3_per_block, d3.m);
+  t.n_z = (d3.m + t.j3_per_block - 1) / t.j3_per_block;
+  t.w3max = 0;
+  for (int z = 0; z < t.n_z; ++z) {
+    const int a = z * t.j3_per_block, b = std::min(a + t.j3_per_block, d3.m) - 1;
+    t.w3max = std::max(t.w3max, d3.h_span[b] - d3.h_span[a] + d3.p + 1);
+  }
+}
+
+// Rows of dimension 1 whose length is not a multiple of 128 bytes make
+// neighbouring tiles share a cache line; keep such tiles on one XCD.
+int default_xcd_map(const Plan& pl) {
+  return (pl.dims[0].m * sizeof(float)) % 128 == 0 ? 0 : 2;
+}
+
+int env_int(const char* name, int fallback) {
+  const char* v = std::getenv(name);
+  if (!v || !*v) return fallback;
+  const int x = std::atoi(v);
+  return x > 0 ? x : fallback;
 }
 
 int checked(hipError_t e, const char* what) {
@@ -172,9 +217,11 @@ int sg_dimension_evaluate(const void* knots_all, int32_t n_knots,
   return checked(e2, "basis kernel");
 }
 
-int sg_plan_create(const sg_grid_desc* desc, void* stream, sg_plan** out) {
-  if (!out) return fail(SG_ERR_INVALID, "plan output pointer is NULL");
-  *out = nullptr;
+}  // extern "C"
+
+namespace sg {
+
+int validate_desc(const sg_grid_desc* desc) {
   if (!desc || !desc->dims) return fail(SG_ERR_INVALID, "grid descriptor is NULL");
   if (desc->n_in < 1 || desc->n_in > SG_MAX_NIN)
     return fail(SG_ERR_INVALID, "n_in must be in 1..8");
@@ -199,10 +246,13 @@ int sg_plan_create(const sg_grid_desc* desc, void* stream, sg_plan** out) {
       if (!(dd.knots_all[k - 1] <= dd.knots_all[k]))
         return fail(SG_ERR_INVALID, os.str() + "knots_all must be sorted");
   }
-  DeviceGuard guard(desc->device);
-  if (guard.err != hipSuccess) return hip_fail(guard.err, "hipSetDevice");
-  hipStream_t s = static_cast<hipStream_t>(stream);
+  return SG_OK;
+}
 
+// Build a plan for a validated descriptor on the current device.  On success
+// *result owns every allocation; on failure everything is freed.
+int build_plan(const sg_grid_desc* desc, hipStream_t s, bool want_adjoint, Plan** result) {
+  *result = nullptr;
   std::unique_ptr<Plan, void (*)(Plan*)> pl(new Plan(), free_plan);
   pl->nin = desc->n_in;
   pl->nout = desc->n_out;
@@ -254,6 +304,22 @@ int sg_plan_create(const sg_grid_desc* desc, void* stream, sg_plan** out) {
   for (int d = 0; d < desc->n_in && rc == SG_OK; ++d) {
     DimDev& D = pl->dims[d];
     D.monotone = std::is_sorted(D.h_span.begin(), D.h_span.end());
+    // power-of-two scale of each derivative order's table (MFMA tiers)
+    const size_t per_order = static_cast<size_t>(D.m) * (D.p + 1);
+    std::vector<double> tab(per_order * (D.D + 1));
+    if (!tab.empty()) {
+      hipError_t e = hipMemcpy(tab.data(), D.table64, tab.size() * sizeof(double),
+                               hipMemcpyDeviceToHost);
+      if (e != hipSuccess) return hip_fail(e, "table copy");
+    }
+    D.tab_exp.assign(D.D + 1, 0);
+    for (int o = 0; o <= D.D; ++o) {
+      double mx = 0.0;
+      for (size_t i = 0; i < per_order; ++i) mx = std::max(mx, std::fabs(tab[o * per_order + i]));
+      int e = 0;
+      if (mx > 0.0) std::frexp(mx, &e);
+      D.tab_exp[o] = e;
+    }
     // transposed band lists: samples whose band holds k, in sample order
     std::vector<int32_t> ptr(D.n + 1, 0), js(static_cast<size_t>(D.m) * (D.p + 1));
     for (int j = 0; j < D.m; ++j)
@@ -276,19 +342,37 @@ int sg_plan_create(const sg_grid_desc* desc, void* stream, sg_plan** out) {
   bool ok = (pl->nin == 2 || pl->nin == 3);
   for (int d = 0; ok && d < pl->nin; ++d) ok = pl->dims[d].monotone && pl->dims[d].m > 0;
   ok = ok && pl->dims[0].p <= kMaxStreamDegree && pl->dims[1].p <= kMaxStreamDegree;
+  pl->nt_stores = env_int("SG_NT_STORES", 0) ? 1 : 0;
   if (ok) {
-    pl->rows_per_wave = 32;
-    compute_tiling(*pl);
-    while (forward_lds_worst(*pl) > static_cast<size_t>(kMaxLds) && pl->rows_per_wave > 1) {
-      pl->rows_per_wave /= 2;
-      compute_tiling(*pl);
+    int rows = env_int("SG_FWD_ROWS", 32);
+    compute_tiling(*pl, pl->fwd, rows);
+    while (forward_lds_worst(*pl) > static_cast<size_t>(kMaxLds) && rows > 1) {
+      rows /= 2;
+      compute_tiling(*pl, pl->fwd, rows);
     }
     ok = forward_lds_worst(*pl) <= static_cast<size_t>(kMaxLds);
+    const char* xm = std::getenv("SG_XCD_MAP");
+    pl->fwd.xcd_map = (xm && *xm) ? std::atoi(xm) : default_xcd_map(*pl);
+  }
+  if (ok && !pl->nurbs && want_adjoint) {
+    int rows = env_int("SG_ADJ_ROWS", 16);
+    const int j3 = env_int("SG_ADJ_J3", 0);
+    compute_tiling(*pl, pl->adj, rows);
+    compute_ztiling(*pl, pl->adj, j3);
+    while (adjoint_lds_min(*pl) > static_cast<size_t>(kMaxLds) && rows > 1) {
+      rows /= 2;
+      compute_tiling(*pl, pl->adj, rows);
+      compute_ztiling(*pl, pl->adj, j3);
+    }
+    if (adjoint_lds_min(*pl) > static_cast<size_t>(kMaxLds)) pl->adj = Tiling();
+    const char* xm = std::getenv("SG_XCD_MAP");
+    pl->adj.xcd_map = (xm && *xm) ? std::atoi(xm) : default_xcd_map(*pl);
   }
   pl->stream_ok = ok;
-  if (ok) {
+  if (ok && pl->adj.n_tiles1 > 0) {
     const DimDev& d1 = pl->dims[0];
     const DimDev& d2 = pl->dims[1];
+    Tiling& tl = pl->adj;
     std::vector<int32_t> t1(2 * d1.n), c2(2 * d2.n);
     for (int k = 0; k < d1.n; ++k) {
       t1[2 * k] = INT_MAX;
@@ -298,28 +382,47 @@ int sg_plan_create(const sg_grid_desc* desc, void* stream, sg_plan** out) {
       c2[2 * k] = INT_MAX;
       c2[2 * k + 1] = -1;
     }
-    for (int t = 0; t < pl->n_tiles1; ++t) {
+    for (int t = 0; t < tl.n_tiles1; ++t) {
       const int a = t * kLanes, b = std::min(a + kLanes, d1.m) - 1;
       for (int k = d1.h_span[a] - d1.p; k <= d1.h_span[b]; ++k) {
         t1[2 * k] = std::min(t1[2 * k], t);
         t1[2 * k + 1] = std::max(t1[2 * k + 1], t);
       }
     }
-    const int chunk = pl->rows_per_wave * kWaves;
-    for (int c = 0; c < pl->n_chunks2; ++c) {
+    const int chunk = tl.chunk_rows();
+    for (int c = 0; c < tl.n_chunks2; ++c) {
       const int a = c * chunk, b = std::min(a + chunk, d2.m) - 1;
       for (int k = d2.h_span[a] - d2.p; k <= d2.h_span[b]; ++k) {
         c2[2 * k] = std::min(c2[2 * k], c);
         c2[2 * k + 1] = std::max(c2[2 * k + 1], c);
       }
     }
-    hipError_t e = dalloc(*pl, &pl->tile_of_k1, t1.size());
-    if (e == hipSuccess) e = dalloc(*pl, &pl->chunk_of_k2, c2.size());
+    std::vector<int32_t> z3;
+    if (pl->nin == 3) {
+      const DimDev& d3 = pl->dims[2];
+      z3.assign(2 * d3.n, 0);
+      for (int k = 0; k < d3.n; ++k) {
+        z3[2 * k] = INT_MAX;
+        z3[2 * k + 1] = -1;
+      }
+      for (int z = 0; z < tl.n_z; ++z) {
+        const int a = z * tl.j3_per_block, b = std::min(a + tl.j3_per_block, d3.m) - 1;
+        for (int k = d3.h_span[a] - d3.p; k <= d3.h_span[b]; ++k) {
+          z3[2 * k] = std::min(z3[2 * k], z);
+          z3[2 * k + 1] = std::max(z3[2 * k + 1], z);
+        }
+      }
+    }
+    hipError_t e = dalloc(*pl, &tl.tile_of_k1, t1.size());
+    if (e == hipSuccess) e = dalloc(*pl, &tl.chunk_of_k2, c2.size());
+    if (e == hipSuccess && !z3.empty()) e = dalloc(*pl, &tl.z_of_k3, z3.size());
+    if (e == hipSuccess && !z3.empty())
+      e = hipMemcpy(tl.z_of_k3, z3.data(), z3.size() * sizeof(int32_t), hipMemcpyHostToDevice);
     if (e == hipSuccess)
-      e = hipMemcpy(pl->tile_of_k1, t1.data(), t1.size() * sizeof(int32_t), hipMemcpyHostToDevice);
+      e = hipMemcpy(tl.tile_of_k1, t1.data(), t1.size() * sizeof(int32_t), hipMemcpyHostToDevice);
     if (e == hipSuccess)
-      e = hipMemcpy(pl->chunk_of_k2, c2.data(), c2.size() * sizeof(int32_t), hipMemcpyHostToDevice);
-    pl->work_bytes = pl->nurbs ? 0 : adjoint_work_bytes(*pl);
+      e = hipMemcpy(tl.chunk_of_k2, c2.data(), c2.size() * sizeof(int32_t), hipMemcpyHostToDevice);
+    pl->work_bytes = adjoint_work_bytes(*pl);
     unsigned char* w = nullptr;
     if (e == hipSuccess) e = dalloc(*pl, &w, pl->work_bytes);
     pl->work = w;
@@ -339,10 +442,30 @@ int sg_plan_create(const sg_grid_desc* desc, void* stream, sg_plan** out) {
     bytes += static_cast<size_t>(D.m) * (D.p + 1) * (D.D + 1) * sizeof(double);
   }
   pl->device_bytes -= bytes;
+  *result = pl.release();
+  return SG_OK;
+}
 
+void destroy_plan(Plan* pl) { free_plan(pl); }
+
+}  // namespace sg
+
+extern "C" {
+
+int sg_plan_create(const sg_grid_desc* desc, void* stream, sg_plan** out) {
+  if (!out) return fail(SG_ERR_INVALID, "plan output pointer is NULL");
+  *out = nullptr;
+  int rc = validate_desc(desc);
+  if (rc != SG_OK) return rc;
+  DeviceGuard guard(desc->device);
+  if (guard.err != hipSuccess) return hip_fail(guard.err, "hipSetDevice");
+  Plan* p = nullptr;
+  rc = build_plan(desc, static_cast<hipStream_t>(stream), true, &p);
+  if (rc != SG_OK) return rc;
   sg_plan* h = new sg_plan();
-  h->p = std::move(*pl);
-  pl->allocations.clear();
+  h->p = std::move(*p);
+  p->allocations.clear();
+  delete p;
   *out = h;
   return SG_OK;
 }
@@ -410,9 +533,25 @@ static int check_common(const sg_plan* plan, int32_t dtype, int32_t tier) {
   if (dtype != SG_F32 && dtype != SG_F64) return fail(SG_ERR_INVALID, "dtype must be SG_F32 or SG_F64");
   if (tier != SG_TIER_NATIVE && tier != SG_TIER_SPLIT_F16 && tier != SG_TIER_BF16)
     return fail(SG_ERR_INVALID, "unknown tier");
-  if (tier != SG_TIER_NATIVE)
-    return fail(SG_ERR_UNSUPPORTED, "this build implements the native tier only");
+  if (tier != SG_TIER_NATIVE) {
+    if (dtype != SG_F32)
+      return fail(SG_ERR_UNSUPPORTED, "the split-FP16 and BF16 tiers take FP32 arrays");
+    if (!mma_supported(plan->p))
+      return fail(SG_ERR_UNSUPPORTED,
+                  "the split-FP16 and BF16 tiers need Nin 2-3, monotone sample points and "
+                  "degrees <= 5 in dimensions 1-2");
+  }
   return SG_OK;
+}
+
+static hipError_t forward(const Plan& pl, int tier, int nf, const int32_t* orders,
+                          const void* P, const void* W, void* const* outs, int dtype,
+                          hipStream_t s) {
+  if (tier == SG_TIER_NATIVE) return launch_forward(pl, nf, orders, P, W, outs, dtype, s);
+  if (n_samples(pl) == 0) return hipSuccess;
+  return launch_forward_mma(pl, tier, nf, orders, static_cast<const float*>(P),
+                            static_cast<const float*>(W), reinterpret_cast<float* const*>(outs),
+                            s);
 }
 
 int sg_evaluate_fields(sg_plan* plan, int32_t n_fields, const int32_t* derivative_orders,
@@ -432,8 +571,8 @@ int sg_evaluate(sg_plan* plan, const int32_t* derivative_order, const void* cont
   DeviceGuard guard(pl.device);
   if (guard.err != hipSuccess) return hip_fail(guard.err, "hipSetDevice");
   void* outs[1] = {eval};
-  return checked(launch_forward(pl, 1, derivative_order, control_points, weights, outs, dtype,
-                                static_cast<hipStream_t>(stream)),
+  return checked(forward(pl, tier, 1, derivative_order, control_points, weights, outs, dtype,
+                         static_cast<hipStream_t>(stream)),
                  "forward launch");
 }
 
@@ -453,8 +592,8 @@ int sg_evaluate_fields(sg_plan* plan, int32_t n_fields, const int32_t* derivativ
     if (!evals[f]) return fail(SG_ERR_INVALID, "null eval pointer");
   DeviceGuard guard(pl.device);
   if (guard.err != hipSuccess) return hip_fail(guard.err, "hipSetDevice");
-  return checked(launch_forward(pl, n_fields, derivative_orders, control_points, nullptr, evals,
-                                dtype, static_cast<hipStream_t>(stream)),
+  return checked(forward(pl, tier, n_fields, derivative_orders, control_points, nullptr, evals,
+                         dtype, static_cast<hipStream_t>(stream)),
                  "forward launch");
 }
 
@@ -465,6 +604,8 @@ int sg_evaluate_adjoint(sg_plan* plan, const int32_t* derivative_order, const vo
   if (rc != SG_OK) return rc;
   const Plan& pl = plan->p;
   if (pl.nurbs) return fail(SG_ERR_INVALID, "Adjoint evaluation not supported for NURBS.");
+  if (tier != SG_TIER_NATIVE)
+    return fail(SG_ERR_UNSUPPORTED, "the adjoint implements the native tier only");
   if ((rc = check_orders(pl, 1, derivative_order)) != SG_OK) return rc;
   if (n_control(pl) == 0) return SG_OK;
   if (!control_points || (!eval && n_samples(pl) > 0))

@@ -23,7 +23,7 @@
 // reference's column-major order) covers every case the streaming kernel
 // does not: Nin = 1 or > 3, degree > 5 in dimensions 1-2, or sample points
 // whose spans are not monotone.
-#include "sg_internal.hpp"
+#include "sg_tiles.hpp"
 
 namespace sg {
 
@@ -31,22 +31,20 @@ namespace {
 
 template <typename T>
 struct StreamArgs {
-  const T* P;
-  const T* W;
+  BandSrc<T> src;                // control points of the tile (stage 1)
   T* out[SG_MAX_FIELDS];
   const T* tab1[SG_MAX_FIELDS];  // dimension 1 at each field's order [m1][p1+1]
   const T* tab2[SG_MAX_FIELDS];  // dimension 2 [m2][P2+1]
-  const T* tab3;                 // dimension 3 (3-D) [m3][p3+1]
   const int32_t* span1;
   const int32_t* span2;
-  const int32_t* span3;
-  int n1, n2;
   int m1, m2;
-  int p1, p3;
+  int p1;
   int c0;                        // first channel handled by this launch
   int nf;                        // fields actually stored (<= NF)
   int rows_per_wave;
-  long long ncp;                 // elements per channel of P
+  int ntx, nty;                  // tiles along dimension 1, chunks along 2
+  int xcd_map;                   // workgroup numbering mode
+  int nt;                        // non-temporal stores of eval
   long long nsamp;               // elements per channel of eval
 };
 
@@ -59,59 +57,17 @@ __global__ __launch_bounds__(kThreads) void fwd_stream(StreamArgs<T> a) {
 
   const int lane = threadIdx.x & (kLanes - 1);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kLanes);
-  const int j1_0 = blockIdx.x * kLanes;
-  const int chunk = a.rows_per_wave * kWaves;
-  const int r0 = blockIdx.y * chunk;
-  const int r1 = min(r0 + chunk, a.m2);
-  const int j3 = (NIN == 3) ? static_cast<int>(blockIdx.z) : 0;
-  const int j1_last = min(j1_0 + kLanes - 1, a.m1 - 1);
+  const FwdTile tile = fwd_tile(a.xcd_map, a.ntx, a.nty, a.rows_per_wave * kWaves, a.m1,
+                                a.m2, NIN);
+  const int j1_0 = tile.j1_0, r0 = tile.r0, r1 = tile.r1, j3 = tile.j3;
   const int lo1 = a.span1[j1_0] - a.p1;
-  const int W1 = a.span1[j1_last] - lo1 + 1;
+  const int W1 = a.span1[tile.j1_last] - lo1 + 1;
   const int lo2 = a.span2[r0] - P2;
   const int W2 = a.span2[r1 - 1] - lo2 + 1;
   const int cs = W1 * W2;  // LDS elements per channel
 
   // ---- stage 1 ----
-  for (int idx = threadIdx.x; idx < cs; idx += kThreads) {
-    const int q2 = idx / W1, q1 = idx - q2 * W1;
-    const long long cpi =
-        static_cast<long long>(lo1 + q1) + static_cast<long long>(a.n1) * (lo2 + q2);
-    T acc[NC];
-    T accw = T(0);
-#pragma unroll
-    for (int c = 0; c < NC; ++c) acc[c] = T(0);
-    if constexpr (NIN == 2) {
-      if constexpr (NURBS) {
-        const T w = a.W[cpi];
-        accw = w;
-#pragma unroll
-        for (int c = 0; c < NC; ++c) acc[c] = w * a.P[cpi + a.ncp * (a.c0 + c)];
-      } else {
-#pragma unroll
-        for (int c = 0; c < NC; ++c) acc[c] = a.P[cpi + a.ncp * (a.c0 + c)];
-      }
-    } else {
-      const int s3 = a.span3[j3];
-      const T* b3 = a.tab3 + static_cast<size_t>(j3) * (a.p3 + 1);
-      const long long plane = static_cast<long long>(a.n1) * a.n2;
-      for (int d = 0; d <= a.p3; ++d) {
-        const long long off = cpi + plane * (s3 - a.p3 + d);
-        const T bd = b3[d];
-        if constexpr (NURBS) {
-          const T bw = bd * a.W[off];
-          accw += bw;
-#pragma unroll
-          for (int c = 0; c < NC; ++c) acc[c] += bw * a.P[off + a.ncp * (a.c0 + c)];
-        } else {
-#pragma unroll
-          for (int c = 0; c < NC; ++c) acc[c] += bd * a.P[off + a.ncp * (a.c0 + c)];
-        }
-      }
-    }
-#pragma unroll
-    for (int c = 0; c < NC; ++c) G[c * cs + idx] = acc[c];
-    if constexpr (NURBS) G[NC * cs + idx] = accw;
-  }
+  stage1_band_window<T, NIN, NC, NURBS>(G, a.src, j3, lo1, W1, lo2, W2);
   __syncthreads();
 
   // ---- stage 2 ----
@@ -193,7 +149,11 @@ __global__ __launch_bounds__(kThreads) void fwd_stream(StreamArgs<T> a) {
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
           const T v = NURBS ? num[c] / num[NCH - 1] : num[c];
-          a.out[f][orow + a.nsamp * c] = v;
+          T* dst = a.out[f] + orow + a.nsamp * c;
+          if (a.nt)
+            __builtin_nontemporal_store(v, dst);
+          else
+            *dst = v;
         }
       }
     }
@@ -336,30 +296,34 @@ hipError_t forward_stream(const Plan& pl, int nf, const int32_t* orders,
   const bool nurbs = pl.nurbs != 0;
   const int nin = pl.nin;
   StreamArgs<T> a{};
-  a.P = P;
-  a.W = W;
+  a.src.P = P;
+  a.src.W = W;
+  a.src.n1 = d1.n;
+  a.src.n2 = d2.n;
   a.span1 = d1.span;
   a.span2 = d2.span;
-  a.n1 = d1.n;
-  a.n2 = d2.n;
   a.m1 = d1.m;
   a.m2 = d2.m;
   a.p1 = d1.p;
-  a.rows_per_wave = pl.rows_per_wave;
-  a.ncp = 1;
+  a.rows_per_wave = pl.fwd.rows_per_wave;
+  a.ntx = pl.fwd.n_tiles1;
+  a.nty = pl.fwd.n_chunks2;
+  a.xcd_map = pl.fwd.xcd_map;
+  a.nt = pl.nt_stores;
+  a.src.ncp = 1;
   a.nsamp = 1;
   for (int d = 0; d < nin; ++d) {
-    a.ncp *= pl.dims[d].n;
+    a.src.ncp *= pl.dims[d].n;
     a.nsamp *= pl.dims[d].m;
   }
   unsigned gz = 1;
   if (nin == 3) {
     const DimDev& d3 = pl.dims[2];
-    a.span3 = d3.span;
-    a.p3 = d3.p;
+    a.src.span3 = d3.span;
+    a.src.p3 = d3.p;
     gz = static_cast<unsigned>(d3.m);
   }
-  const dim3 grid(pl.n_tiles1, pl.n_chunks2, gz);
+  const dim3 grid(pl.fwd.n_tiles1 * pl.fwd.n_chunks2 * gz);
 
   // Fields that share the dimension-3 order share stage 1; launch each group
   // in blocks of up to 4 fields (B-splines) and each channel block.
@@ -381,7 +345,7 @@ hipError_t forward_stream(const Plan& pl, int nf, const int32_t* orders,
       a.tab2[g] = ktab<T>(d2, orders[f * nin + 1]);
       a.out[g] = outs[f];
     }
-    if (nin == 3) a.tab3 = ktab<T>(pl.dims[2], orders[group[0] * nin + 2]);
+    if (nin == 3) a.src.tab3 = ktab<T>(pl.dims[2], orders[group[0] * nin + 2]);
     a.nf = gnf;
     int c0 = 0;
     while (c0 < pl.nout) {
@@ -392,8 +356,9 @@ hipError_t forward_stream(const Plan& pl, int nf, const int32_t* orders,
                                      : pick_stream<T, 3>(d2.p, kf, nc, nurbs);
       if (!k) return hipErrorInvalidValue;
       a.c0 = c0;
-      const size_t lds = static_cast<size_t>(nc + (nurbs ? 1 : 0)) * pl.w1max *
-                         pl.w2max * sizeof(T);
+      a.src.c0 = c0;
+      const size_t lds = static_cast<size_t>(nc + (nurbs ? 1 : 0)) * pl.fwd.w1max *
+                         pl.fwd.w2max * sizeof(T);
       hipLaunchKernelGGL(k, grid, dim3(kThreads), lds, s, a);
       hipError_t e = hipGetLastError();
       if (e != hipSuccess) return e;

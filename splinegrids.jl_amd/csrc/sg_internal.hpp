@@ -30,32 +30,75 @@ struct DimDev {
   double* ktab64 = nullptr;   // [D+1][m][p+1]: the p+1 values of a sample
   float* ktab32 = nullptr;    //   are contiguous (kernel layout)
   std::vector<int32_t> h_span;
+  std::vector<int> tab_exp;   // per order: max|B| * 2^-e in [0.5, 1)
   bool monotone = true;       // spans non-decreasing in the sample index
-  // For every basis function k: first and last sample whose band holds k
-  // (last < first if none); only meaningful when monotone.
-  int32_t* supp = nullptr;    // [n][2]
   // Transposed band lists for the general adjoint: samples j whose band
   // holds k, with their slot a = k - (s_j - p).
   int32_t* csr_ptr = nullptr; // [n + 1]
   int32_t* csr_js = nullptr;  // [m * (p+1)]: j * (p + 1) + a
 };
 
+// Tiling of the sample grid for the streaming kernels: tiles of kLanes
+// samples along dimension 1, chunks of kWaves * rows_per_wave rows along
+// dimension 2, one workgroup per (tile, chunk, sample of dimension 3).
+struct Tiling {
+  int rows_per_wave = 32;
+  int n_tiles1 = 0, n_chunks2 = 0;
+  int w1max = 0, w2max = 0;   // largest band windows of a tile / chunk
+  int w2wave_max = 0;         // largest window of one wave's rows
+  int xcd_map = 0;            // workgroup numbering, see xcd_logical_block
+  // adjoint only: samples of dimension 3 per workgroup and their k3 windows
+  int j3_per_block = 1, n_z = 1, w3max = 1;
+  // tile / chunk / z-range ranges whose windows hold each k1 / k2 / k3
+  int32_t* tile_of_k1 = nullptr;   // [n1][2]
+  int32_t* chunk_of_k2 = nullptr;  // [n2][2]
+  int32_t* z_of_k3 = nullptr;      // [n3][2]
+  int chunk_rows() const { return rows_per_wave * kWaves; }
+};
+
 struct Plan {
   int nin = 0, nout = 0, nurbs = 0, device = 0;
   std::vector<DimDev> dims;
   bool stream_ok = false;     // streaming (sum-factorised) kernels apply
-  int rows_per_wave = 32;     // rows of dimension 2 per wave
-  int n_tiles1 = 0, n_chunks2 = 0;
-  int w1max = 0, w2max = 0;   // largest band windows of a tile / chunk
-  int w2wave_max = 0;         // largest window of one wave's rows
-  // adjoint: tile / chunk ranges holding each k1 / k2
-  int32_t* tile_of_k1 = nullptr;   // [n1][2]
-  int32_t* chunk_of_k2 = nullptr;  // [n2][2]
-  void* work = nullptr;            // adjoint partial slabs
+  Tiling fwd, adj;
+  int nt_stores = 0;          // non-temporal stores of eval
+  void* work = nullptr;       // adjoint partial slabs
   size_t work_bytes = 0;
   size_t device_bytes = 0;
   std::vector<void*> allocations;
 };
+
+// Workgroups are dispatched round-robin over the 8 XCDs of an MI355X, each
+// with its own L2.  When rows of eval are not 128-byte aligned, neighbouring
+// tiles of a row share the cache line at their edge; numbering the 1-D grid
+// so that such tiles run on one XCD lets that line be fetched (or merged)
+// once.  Modes (each a bijection of [0, gridDim.x)):
+//   0  logical = physical (consecutive tiles on consecutive XCDs)
+//   1  every XCD takes one contiguous range of logical blocks
+//   2  groups of G consecutive logical blocks (one row of tiles) go to one
+//      XCD, groups dealt round-robin over the XCDs
+constexpr int kXcds = 8;
+__device__ inline int xcd_logical_block(int mode, int G) {
+  const int n = static_cast<int>(gridDim.x);
+  const int p = static_cast<int>(blockIdx.x);
+  const int xcd = p % kXcds, local = p / kXcds;
+  if (mode == 1) {
+    const int q = n / kXcds, r = n % kXcds;
+    return xcd < r ? xcd * (q + 1) + local : r * (q + 1) + (xcd - r) * q + local;
+  }
+  if (mode == 2 && G > 0) {
+    const int ng = n / G;
+    const int nmain = (ng / kXcds) * kXcds * G;
+    if (p >= nmain) return p;
+    return (xcd + kXcds * (local / G)) * G + local % G;
+  }
+  return p;
+}
+
+// ---- plans (sg_api.cpp) ----
+int validate_desc(const sg_grid_desc* desc);
+int build_plan(const sg_grid_desc* desc, hipStream_t s, bool want_adjoint, Plan** result);
+void destroy_plan(Plan* pl);
 
 // ---- error reporting (sg_api.cpp) ----
 void set_error(const std::string& msg);
@@ -84,10 +127,17 @@ hipError_t launch_forward(const Plan& plan, int nf, const int32_t* orders,
                           const void* P, const void* W, void* const* outs,
                           int dtype, hipStream_t s);
 
+// ---- sg_mfma.hip ----
+bool mma_supported(const Plan& plan);
+hipError_t launch_forward_mma(const Plan& plan, int tier, int nf, const int32_t* orders,
+                              const float* P, const float* W, float* const* outs,
+                              hipStream_t s);
+
 // ---- sg_adjoint.hip ----
 hipError_t launch_adjoint(const Plan& plan, const int32_t* orders,
                           const void* ebar, void* Pbar, int dtype,
                           int accumulate, hipStream_t s);
 size_t adjoint_work_bytes(const Plan& plan);
+size_t adjoint_stream_lds(const Plan& plan, int nc, size_t elem);
 
 }  // namespace sg

@@ -1,0 +1,311 @@
+// sg_mfma.hip -- evaluate!(grid) with the contraction on MFMA (FP32 arrays).
+//
+// Tiers
+//   SPLIT_F16  every operand x is split into hi = fp16(x) and lo = fp16(x-hi)
+//              after power-of-two scaling; products use hi*hi + hi*lo + lo*hi
+//              on v_mfma_f32_16x16x16_f16 with FP32 accumulation (the lo*lo
+//              term is below 2^-22 relative).
+//   BF16       one pass of v_mfma_f32_16x16x16bf16_1k on bf16(x).
+//
+// Structure (per workgroup, same tiling and stage 1 as fwd_stream): the
+// tile's band window G (FP32, LDS) is scaled by 2^-eG so that max|G| < 1.
+// Each wave owns 16 consecutive samples j1 of the 64-sample tile and walks
+// the chunk's rows in blocks of 16.  For a block, with K = 16 consecutive
+// control points k2 from kb,
+//   V^T[k2][j1]  = sum_a B1[j1, a] * G[k2][s1-p1+a]        (FP32 FMA)
+//   Out[j2][j1]  = sum_k2 B2d[j2][k2] * V^T[k2][j1]         (MFMA, 16x16x16)
+// where B2d is the banded B2 made dense on the fly.  V^T is kept while the
+// rows' bands stay inside [kb, kb+16).  MFMA operand layout (gfx9 16x16x16):
+// lane l = 16 g + i holds A[i][4g..4g+3], B[4g..4g+3][i], D[4g..4g+3][i].
+#include "sg_tiles.hpp"
+
+namespace sg {
+
+namespace {
+
+typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
+typedef short short4_t __attribute__((ext_vector_type(4)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+struct MmaArgs {
+  BandSrc<float> src;
+  float* out;
+  const float* tab1;  // [m1][p1+1]
+  const float* tab2;  // [m2][p2+1]
+  const int32_t* span1;
+  const int32_t* span2;
+  int m1, m2, p1, p2;
+  int rows_per_chunk;
+  int ntx, nty, xcd_map;
+  float s1, s2;        // 2^-e of dimension 1 / 2 tables at the requested orders
+  int e12;             // e1 + e2
+  long long nsamp;
+};
+
+__device__ inline short bf16_bits(float x) {  // round to nearest even
+  unsigned u = __float_as_uint(x);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return static_cast<short>(u >> 16);
+}
+
+template <int TIER>
+struct Operand;
+
+template <>
+struct Operand<SG_TIER_SPLIT_F16> {
+  half4_t hi, lo;
+  __device__ void set(int j, float x) {
+    const _Float16 h = static_cast<_Float16>(x);
+    hi[j] = h;
+    lo[j] = static_cast<_Float16>(x - static_cast<float>(h));
+  }
+  __device__ static float4_t mma(const Operand& a, const Operand& b, float4_t c) {
+    c = __builtin_amdgcn_mfma_f32_16x16x16f16(a.hi, b.hi, c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_16x16x16f16(a.hi, b.lo, c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_16x16x16f16(a.lo, b.hi, c, 0, 0, 0);
+    return c;
+  }
+};
+
+template <>
+struct Operand<SG_TIER_BF16> {
+  short4_t v;
+  __device__ void set(int j, float x) { v[j] = bf16_bits(x); }
+  __device__ static float4_t mma(const Operand& a, const Operand& b, float4_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a.v, b.v, c, 0, 0, 0);
+  }
+};
+
+template <int TIER, int NIN, int NC, bool NURBS>
+__global__ __launch_bounds__(kThreads) void mma_stream(MmaArgs a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int NCH = NC + (NURBS ? 1 : 0);
+  constexpr int PM = kMaxStreamDegree;
+  float* G = reinterpret_cast<float*>(smem);
+  __shared__ float red[kWaves][NCH];
+
+  const int lane = threadIdx.x & (kLanes - 1);
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kLanes);
+  const FwdTile tile = fwd_tile(a.xcd_map, a.ntx, a.nty, a.rows_per_chunk, a.m1, a.m2, NIN);
+  const int r0 = tile.r0, r1 = tile.r1, j3 = tile.j3;
+  const int p1 = a.p1, p2 = a.p2;
+  const int lo1 = a.span1[tile.j1_0] - p1;
+  const int W1 = a.span1[tile.j1_last] - lo1 + 1;
+  const int lo2 = a.span2[r0] - p2;
+  const int hi2 = a.span2[r1 - 1];
+  const int W2 = hi2 - lo2 + 1;
+  const int cs = W1 * W2;
+
+  stage1_band_window<float, NIN, NC, NURBS>(G, a.src, j3, lo1, W1, lo2, W2);
+  __syncthreads();
+
+  // per-channel power-of-two scale of the window: max|G_c| * 2^-e < 1
+  float mx[NCH];
+#pragma unroll
+  for (int ch = 0; ch < NCH; ++ch) {
+    float m = 0.f;
+    for (int i = threadIdx.x; i < cs; i += kThreads) m = fmaxf(m, fabsf(G[ch * cs + i]));
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+    mx[ch] = m;
+  }
+  if (lane == 0)
+#pragma unroll
+    for (int ch = 0; ch < NCH; ++ch) red[wave][ch] = mx[ch];
+  __syncthreads();
+  float gs[NCH];  // 2^-eG
+  int eg[NCH];
+#pragma unroll
+  for (int ch = 0; ch < NCH; ++ch) {
+    float m = fmaxf(fmaxf(red[0][ch], red[1][ch]), fmaxf(red[2][ch], red[3][ch]));
+    int e = 0;
+    if (m > 0.f) frexpf(m, &e);
+    eg[ch] = e;
+    gs[ch] = ldexpf(1.f, -e);
+  }
+
+  // ---- per-lane constants: sample j1 (column) and k-quarter g ----
+  const int g = lane >> 4, col = lane & 15;
+  const int j1q = tile.j1_0 + 16 * wave + col;
+  const bool active = j1q < a.m1;
+  const int j1c = active ? j1q : a.m1 - 1;
+  const int q1b = a.span1[j1c] - p1 - lo1;
+  float b1[PM + 1];
+#pragma unroll
+  for (int t = 0; t <= PM; ++t)
+    b1[t] = (t <= p1) ? a.tab1[static_cast<size_t>(j1c) * (p1 + 1) + min(t, p1)] * a.s1 : 0.f;
+
+  Operand<TIER> vop[NCH];
+  int kb = -(1 << 30);  // k2 base of the cached V^T operand
+  auto build_v = [&](int base) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int k2 = base + 4 * g + j;
+      const bool in = k2 >= lo2 && k2 <= hi2;
+      const float* gr = G + (in ? (k2 - lo2) : 0) * W1 + q1b;
+#pragma unroll
+      for (int ch = 0; ch < NCH; ++ch) {
+        float v = 0.f;
+#pragma unroll
+        for (int t = 0; t <= PM; ++t)
+          if (t <= p1) v += b1[t] * gr[ch * cs + min(t, p1)];
+        vop[ch].set(j, in ? v * gs[ch] : 0.f);
+      }
+    }
+  };
+
+  const long long obase = static_cast<long long>(j1q) + static_cast<long long>(a.m1) * a.m2 * j3 +
+                          a.nsamp * a.src.c0;
+  for (int jb = r0; jb < r1; jb += 16) {
+    const int jl = min(jb + 15, r1 - 1);
+    const int klo = a.span2[jb] - p2, khi = a.span2[jl];
+    // this lane's row of the A operand
+    const bool rowok = jb + col < r1;
+    const int j2a = rowok ? jb + col : r1 - 1;
+    const int base2 = a.span2[j2a] - p2;
+    float b2[PM + 1];
+#pragma unroll
+    for (int t = 0; t <= PM; ++t)
+      b2[t] = (rowok && t <= p2) ? a.tab2[static_cast<size_t>(j2a) * (p2 + 1) + min(t, p2)] * a.s2
+                                 : 0.f;
+    float4_t acc[NCH];
+#pragma unroll
+    for (int ch = 0; ch < NCH; ++ch) acc[ch] = float4_t{0.f, 0.f, 0.f, 0.f};
+    int kc = (klo >= kb && khi <= kb + 15) ? kb : klo;
+    for (; kc <= khi; kc += 16) {
+      if (kc != kb) {
+        kb = kc;
+        build_v(kb);
+      }
+      Operand<TIER> aop;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int off = kc + 4 * g + j - base2;
+        float v = 0.f;
+#pragma unroll
+        for (int t = 0; t <= PM; ++t)
+          if (off == t) v = b2[t];
+        aop.set(j, v);
+      }
+#pragma unroll
+      for (int ch = 0; ch < NCH; ++ch) acc[ch] = Operand<TIER>::mma(aop, vop[ch], acc[ch]);
+    }
+    if (active) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int j2 = jb + 4 * g + j;
+        if (j2 >= r1) continue;
+        const long long o = obase + static_cast<long long>(a.m1) * j2;
+        if constexpr (NURBS) {
+          const float den = ldexpf(acc[NC][j], eg[NC] + a.e12);
+#pragma unroll
+          for (int c = 0; c < NC; ++c)
+            a.out[o + a.nsamp * c] = ldexpf(acc[c][j], eg[c] + a.e12) / den;
+        } else {
+#pragma unroll
+          for (int c = 0; c < NC; ++c)
+            a.out[o + a.nsamp * c] = ldexpf(acc[c][j], eg[c] + a.e12);
+        }
+      }
+    }
+  }
+}
+
+using MmaKernel = void (*)(MmaArgs);
+
+template <int TIER, int NIN>
+MmaKernel pick_mma_nc(int nc, bool nurbs) {
+  if (nurbs) {
+    switch (nc) {
+      case 1: return mma_stream<TIER, NIN, 1, true>;
+      case 2: return mma_stream<TIER, NIN, 2, true>;
+      case 3: return mma_stream<TIER, NIN, 3, true>;
+    }
+    return nullptr;
+  }
+  switch (nc) {
+    case 1: return mma_stream<TIER, NIN, 1, false>;
+    case 2: return mma_stream<TIER, NIN, 2, false>;
+    case 3: return mma_stream<TIER, NIN, 3, false>;
+    case 4: return mma_stream<TIER, NIN, 4, false>;
+  }
+  return nullptr;
+}
+
+MmaKernel pick_mma(int tier, int nin, int nc, bool nurbs) {
+  if (tier == SG_TIER_SPLIT_F16)
+    return nin == 2 ? pick_mma_nc<SG_TIER_SPLIT_F16, 2>(nc, nurbs)
+                    : pick_mma_nc<SG_TIER_SPLIT_F16, 3>(nc, nurbs);
+  return nin == 2 ? pick_mma_nc<SG_TIER_BF16, 2>(nc, nurbs)
+                  : pick_mma_nc<SG_TIER_BF16, 3>(nc, nurbs);
+}
+
+}  // namespace
+
+bool mma_supported(const Plan& pl) { return pl.stream_ok; }
+
+hipError_t launch_forward_mma(const Plan& pl, int tier, int nf, const int32_t* orders,
+                              const float* P, const float* W, float* const* outs,
+                              hipStream_t s) {
+  const DimDev& d1 = pl.dims[0];
+  const DimDev& d2 = pl.dims[1];
+  const int nin = pl.nin;
+  const bool nurbs = pl.nurbs != 0;
+  MmaArgs a{};
+  a.src.P = P;
+  a.src.W = W;
+  a.src.n1 = d1.n;
+  a.src.n2 = d2.n;
+  a.span1 = d1.span;
+  a.span2 = d2.span;
+  a.m1 = d1.m;
+  a.m2 = d2.m;
+  a.p1 = d1.p;
+  a.p2 = d2.p;
+  a.rows_per_chunk = pl.fwd.chunk_rows();
+  a.ntx = pl.fwd.n_tiles1;
+  a.nty = pl.fwd.n_chunks2;
+  a.xcd_map = pl.fwd.xcd_map;
+  a.src.ncp = 1;
+  a.nsamp = 1;
+  for (int d = 0; d < nin; ++d) {
+    a.src.ncp *= pl.dims[d].n;
+    a.nsamp *= pl.dims[d].m;
+  }
+  unsigned gz = 1;
+  if (nin == 3) {
+    a.src.span3 = pl.dims[2].span;
+    a.src.p3 = pl.dims[2].p;
+    gz = static_cast<unsigned>(pl.dims[2].m);
+  }
+  const dim3 grid(pl.fwd.n_tiles1 * pl.fwd.n_chunks2 * gz);
+  for (int f = 0; f < nf; ++f) {
+    const int o1 = orders[f * nin + 0], o2 = orders[f * nin + 1];
+    a.tab1 = d1.ktab32 + static_cast<size_t>(o1) * d1.m * (d1.p + 1);
+    a.tab2 = d2.ktab32 + static_cast<size_t>(o2) * d2.m * (d2.p + 1);
+    if (nin == 3)
+      a.src.tab3 = pl.dims[2].ktab32 +
+                   static_cast<size_t>(orders[f * nin + 2]) * pl.dims[2].m * (pl.dims[2].p + 1);
+    const int e1 = d1.tab_exp[o1], e2 = d2.tab_exp[o2];
+    a.s1 = ldexpf(1.f, -e1);
+    a.s2 = ldexpf(1.f, -e2);
+    a.e12 = e1 + e2;
+    a.out = outs[f];
+    for (int c0 = 0; c0 < pl.nout;) {
+      const int left = pl.nout - c0;
+      const int nc = nurbs ? std::min(left, 3) : std::min(left, 4);
+      MmaKernel k = pick_mma(tier, nin, nc, nurbs);
+      if (!k) return hipErrorInvalidValue;
+      a.src.c0 = c0;
+      const size_t lds = static_cast<size_t>(nc + (nurbs ? 1 : 0)) * pl.fwd.w1max *
+                         pl.fwd.w2max * sizeof(float);
+      hipLaunchKernelGGL(k, grid, dim3(kThreads), lds, s, a);
+      hipError_t e = hipGetLastError();
+      if (e != hipSuccess) return e;
+      c0 += nc;
+    }
+  }
+  return hipSuccess;
+}
+
+}  // namespace sg
