@@ -77,6 +77,10 @@ def workload(name: str):
         return dict(name=f"{name}: 1 GiB {what} (bandwidth and counter calibration)",
                     n=(1,), p=(0,), m=(1 << 28,), D=(0,), nout=1, dtype=np.float32,
                     knots=None, fields=[(0,)], P=None, W=None, copy=name)
+    if name == "c5":
+        return dict(name="c5: 1024 uneven grids (half 2-D, half 3-D, 25 % NURBS, Nout 1|3), "
+                         "one persistent launch, hipGraph, FP32",
+                    batch=True, dtype=np.float32, fields=[None], W=None)
     if name in ("c4", "c4a"):
         rng = np.random.default_rng(SEED0 + 4)
         P = rng.uniform(-1, 1, (40, 40, 20, 3)).astype(np.float32)
@@ -86,6 +90,40 @@ def workload(name: str):
                     dtype=np.float32, knots=None, fields=[(0, 0, 0)], P=P, W=None,
                     adjoint=(name == "c4a"))
     raise ValueError(f"unknown workload {name}")
+
+
+def c5_specs(rng, n_grids=1024, small=False):
+    """The uneven batch: half 2-D, half 3-D grids; control points per dimension
+    U{8..64}; degree U{1..4} capped at n-1; samples per dimension U{32..512}
+    (2-D) or U{16..128} (3-D) (smaller ranges with small=True, for tests);
+    Nout in {1, 3}; 25 % NURBS."""
+    specs = []
+    for i in range(n_grids):
+        nin = 2 if i < n_grids // 2 else 3
+        n = tuple(int(x) for x in rng.integers(8, 65, nin))
+        p = tuple(int(min(rng.integers(1, 5), nd - 1)) for nd in n)
+        lo, hi = ((32, 513) if nin == 2 else (16, 129)) if not small else \
+            ((16, 81) if nin == 2 else (8, 25))
+        m = tuple(int(x) for x in rng.integers(lo, hi, nin))
+        specs.append(dict(n=n, p=p, m=m, nout=int(rng.choice([1, 3])),
+                          nurbs=bool(rng.random() < 0.25)))
+    return specs
+
+
+def build_spec_grid(sg, spec, rng, dtype=np.float32, device="cuda"):
+    """A grid of one batch spec with P ~ U[-1, 1) and W ~ U[0.5, 2]."""
+    dims = []
+    for nd, pd, md in zip(spec["n"], spec["p"], spec["m"]):
+        kw = {}
+        if spec.get("unsorted"):
+            kw["sample_points"] = rng.permutation(np.linspace(0.0, 1.0, md))
+        dims.append(sg.SplineDimension(nd, pd, md, float_type=dtype, device=device, **kw))
+    grid = (sg.NURBSGrid if spec["nurbs"] else sg.SplineGrid)(tuple(dims), spec["nout"])
+    grid.control_points.copy_(rng.uniform(-1, 1, tuple(spec["n"]) + (spec["nout"],)).astype(dtype))
+    if spec["nurbs"]:
+        grid.weights.copy_(sg.from_numpy_colmajor(
+            rng.uniform(0.5, 2.0, spec["n"]).astype(dtype), grid.dtype, device))
+    return grid
 
 
 def build_grid(sg, wl, device="cuda"):
@@ -225,6 +263,75 @@ def copy_calibration(args, wl):
     return 0
 
 
+def batch_bytes(grids) -> int:
+    total = 0
+    for g in grids:
+        nsamp = math.prod(s.n_sample_points for s in g.spline_dimensions)
+        ncp = math.prod(s.n_basis_functions for s in g.spline_dimensions)
+        total += 4 * (nsamp * g.Nout + ncp * g.Nout + (ncp if g.is_nurbs else 0))
+        total += sum(s.n_sample_points * (s.degree + 1) * 4 + 4 * s.n_sample_points
+                     for s in g.spline_dimensions)
+    return total
+
+
+def bench_batch(args, sg, wl):
+    """C5: the per-grid launch loop, the batched persistent launch, and the
+    batched launch captured in a hipGraph; value is the graph replay."""
+    import torch
+
+    rng = np.random.default_rng(SEED0 + 5)
+    specs = c5_specs(rng)
+    t0 = time.perf_counter()
+    grids = [build_spec_grid(sg, spec, rng) for spec in specs]
+    batch = sg.SplineGridBatch(grids)
+    setup_s = time.perf_counter() - t0
+    info = batch.autotune(reps=3)
+
+    def timed(fn, steps):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(steps):
+            fn()
+        b.record()
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / steps
+
+    steps = max(5, args.steps // 10)
+    ms_loop = timed(batch.evaluate_each_, steps)
+    ms_batch = timed(batch.evaluate_, steps)
+    stream = torch.cuda.Stream()
+    stream.wait_stream(torch.cuda.current_stream())
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(stream):
+        batch.evaluate_()
+        torch.cuda.synchronize()
+        with torch.cuda.graph(graph, stream=stream):
+            batch.evaluate_()
+    torch.cuda.synchronize()
+    ms_graph = timed(graph.replay, steps)
+    nsamp = info["samples"]
+    B = batch_bytes(grids)
+    out = {
+        "metric": METRIC, "value": nsamp / (ms_graph * 1e-3), "unit": "sample-pts/s",
+        "n_gpus": 1, "steps": steps, "warmup": args.warmup, "ms_per_step": ms_graph,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded inputs of the workload's shape)",
+        "config": {"workload": wl["name"], "grids": len(grids), **info,
+                   "setup_s": setup_s},
+        "roofline": {"bound": "hbm", "achieved": B / (ms_graph * 1e-3) / 1e9,
+                     "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": B / (ms_graph * 1e-3) / 1e9 / PEAK_HBM_GBS, "traffic": None,
+                     "algorithmic_bytes_per_step": B},
+        "per_grid_loop_ms": ms_loop, "batched_ms": ms_batch, "graph_ms": ms_graph,
+        "speedup_vs_per_grid_loop": ms_loop / ms_graph,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -251,6 +358,8 @@ def main(argv=None):
     wl = workload(args.workload)
     if wl.get("copy"):
         return copy_calibration(args, wl)
+    if wl.get("batch"):
+        return bench_batch(args, sg, wl)
     grid = build_grid(sg, wl)
     adjoint = wl.get("adjoint", False)
     fields = wl["fields"]

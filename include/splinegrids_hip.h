@@ -140,6 +140,43 @@ int sg_evaluate_adjoint(sg_plan* plan, const int32_t* derivative_order,
 /* synchronize(backend) after a launch (e.g. src/spline_grid.jl:228). */
 int sg_stream_synchronize(void* stream);
 
+/* ---- batches: many independent grids in one persistent launch ----------
+ * No counterpart in the reference, which evaluates one grid per evaluate!
+ * call; the batch is for callers that hold many small grids (the per-grid
+ * loop of evaluate! calls is what it replaces).  Every grid is evaluated
+ * with derivative order 0 on FP32 arrays. */
+typedef struct sg_batch sg_batch;
+
+/* Build per-grid FP64 tables and one list of tiles over all grids, sorted
+ * largest first; allocates all device memory of the batch. */
+int sg_batch_create(const sg_grid_desc* grids, int32_t n_grids, void* stream,
+                    sg_batch** batch);
+int sg_batch_destroy(sg_batch* batch);
+
+/* Device pointers of every grid's arrays (weights[i] NULL for B-spline
+ * grids, or weights itself NULL); kept until set again. */
+int sg_batch_set_arrays(sg_batch* batch, const void* const* control_points,
+                        const void* const* weights, void* const* evals);
+
+/* Enqueue the evaluation of every grid: a one-thread kernel that resets the
+ * work-queue counter, then the persistent kernel.  Only kernel launches, so
+ * the call can be captured into a hipGraph and replayed. */
+int sg_batch_evaluate(sg_batch* batch, int32_t tier, void* stream);
+
+/* The loop that a batch replaces, for comparison: one sg_evaluate-style
+ * launch per grid (native tier) on `stream`. */
+int sg_batch_evaluate_each(sg_batch* batch, void* stream);
+
+/* Time candidate tilings (rows per wave, persistent workgroups per CU) on
+ * the arrays set last and keep the fastest; `reps` timed launches each. */
+int sg_batch_autotune(sg_batch* batch, int32_t reps, void* stream);
+
+/* Totals of the batch: tiles, sample points, grids evaluated by the
+ * persistent kernel, and the tuned rows per wave / workgroups per CU. */
+int sg_batch_info(const sg_batch* batch, int64_t* n_tiles, int64_t* n_samples,
+                  int32_t* n_persistent, int32_t* rows_per_wave,
+                  int32_t* blocks_per_cu);
+
 #ifdef __cplusplus
 }
 #endif

@@ -15,6 +15,7 @@ from .spline_grid import (NURBSGrid, Plan, SplineGrid, evaluate_, evaluate_adjoi
 from .validation import validate_partial_derivatives, validate_spline_grid
 from .autograd import SplineEval, spline_eval
 from .linear_map import LinearMap
+from .batch import SplineGridBatch
 from .utils import colmajor, colmajor_empty, colmajor_zeros, from_numpy_colmajor, to_numpy_colmajor
 
 __version__ = "0.1.0"
@@ -26,4 +27,5 @@ __all__ = [
     "reshape_colmajor", "vec_colmajor", "validate_partial_derivatives",
     "validate_spline_grid", "colmajor", "colmajor_empty", "colmajor_zeros",
     "from_numpy_colmajor", "to_numpy_colmajor", "SplineEval", "spline_eval", "LinearMap",
+    "SplineGridBatch",
 ]

@@ -22,6 +22,8 @@ EXPORTS = (
     "sg_last_error", "sg_version", "sg_dimension_evaluate", "sg_plan_create",
     "sg_plan_destroy", "sg_plan_basis", "sg_plan_device_bytes", "sg_evaluate",
     "sg_evaluate_fields", "sg_evaluate_adjoint", "sg_stream_synchronize",
+    "sg_batch_create", "sg_batch_destroy", "sg_batch_set_arrays", "sg_batch_evaluate",
+    "sg_batch_evaluate_each", "sg_batch_autotune", "sg_batch_info",
 )
 
 
@@ -79,6 +81,15 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
         "sg_evaluate_fields": (ctypes.c_int, [vp, i32, i32p, vp, ctypes.POINTER(vp), i32, i32, vp]),
         "sg_evaluate_adjoint": (ctypes.c_int, [vp, i32p, vp, vp, i32, i32, i32, vp]),
         "sg_stream_synchronize": (ctypes.c_int, [vp]),
+        "sg_batch_create": (ctypes.c_int, [ctypes.POINTER(GridDesc), i32, vp, ctypes.POINTER(vp)]),
+        "sg_batch_destroy": (ctypes.c_int, [vp]),
+        "sg_batch_set_arrays": (ctypes.c_int, [vp, ctypes.POINTER(vp), ctypes.POINTER(vp),
+                                               ctypes.POINTER(vp)]),
+        "sg_batch_evaluate": (ctypes.c_int, [vp, i32, vp]),
+        "sg_batch_evaluate_each": (ctypes.c_int, [vp, vp]),
+        "sg_batch_autotune": (ctypes.c_int, [vp, i32, vp]),
+        "sg_batch_info": (ctypes.c_int, [vp, ctypes.POINTER(i64), ctypes.POINTER(i64), i32p,
+                                         i32p, i32p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

@@ -74,17 +74,28 @@ __device__ inline double dot_lanes(const double* a, const double* b, int l0, int
 
 constexpr int kRowBatch = 4;  // rows whose loads are in flight together
 
+// LDS rows of 64 lanes are padded by 16 bytes so that rows read by
+// neighbouring threads start in different banks.
+template <typename T>
+constexpr int lane_stride() { return kLanes + 16 / static_cast<int>(sizeof(T)); }
+
 template <typename T, int P2, int NC>
 __global__ __launch_bounds__(kThreads) void adj_stream(AdjStreamArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem[];
-  // S   [4][w2wave][NC][64]  finished k2 entries of each wave (one j3)
-  // BlT [w1max][64]          B1 of lane l at k1 = lo1 + q1 (0 off the band)
+  // S   [4][w2wave][NC][LS]  finished k2 entries of each wave (one j3)
+  // BlT [w1max][LS]          B1 of lane l at k1 = lo1 + q1 (0 off the band)
   // R3  [p3+1][w1max*w2max*NC] ring of k3-slabs
+  constexpr int LS = lane_stride<T>();
   T* S = reinterpret_cast<T*>(smem);
-  T* BlT = S + static_cast<size_t>(kWaves) * a.w2wave * NC * kLanes;
-  T* R3 = BlT + static_cast<size_t>(a.w1max) * kLanes;
+  T* BlT = S + static_cast<size_t>(kWaves) * a.w2wave * NC * LS;
+  T* R3 = BlT + static_cast<size_t>(a.w1max) * LS;
   const int slab = a.w1max * a.w2max * NC;
-  int* wlo = reinterpret_cast<int*>(R3 + static_cast<size_t>(a.p3 + 1) * slab);
+  // RT [chunk][P2+1] B2 of the chunk's rows; RS [chunk] their spans (LDS
+  // reads keep vector-memory loads of tables out of the row loop)
+  T* RT = R3 + static_cast<size_t>(a.p3 + 1) * slab;
+  const int chunk_cap = a.rows_per_wave * kWaves;
+  int* RS = reinterpret_cast<int*>(RT + static_cast<size_t>(chunk_cap) * (P2 + 1));
+  int* wlo = RS + chunk_cap;
   int* whi = wlo + kWaves;
   int* llo = whi + kWaves;   // [w1max]
   int* lhi = llo + a.w1max;  // [w1max]
@@ -119,7 +130,7 @@ __global__ __launch_bounds__(kThreads) void adj_stream(AdjStreamArgs<T> a) {
       const int off = lo1 + q1 - (a.span1[jl] - p1);  // slot of k1 in lane l's band
       if (off >= 0 && off <= p1) v = a.tab1[static_cast<size_t>(jl) * (p1 + 1) + off];
     }
-    BlT[i] = v;
+    BlT[q1 * LS + l] = v;
   }
   // lanes whose band holds k1: s1 in [k1, k1 + p1] (spans monotone in lanes)
   for (int q1 = threadIdx.x; q1 < W1; q1 += kThreads) {
@@ -138,6 +149,9 @@ __global__ __launch_bounds__(kThreads) void adj_stream(AdjStreamArgs<T> a) {
     lhi[q1] = (lo_b + 3) & ~3;
   }
   for (int i = threadIdx.x; i < (p3 + 1) * slab; i += kThreads) R3[i] = T(0);
+  for (int i = threadIdx.x; i < r1 - r0; i += kThreads) RS[i] = a.span2[r0 + i];
+  for (int i = threadIdx.x; i < (r1 - r0) * (P2 + 1); i += kThreads)
+    RT[i] = a.tab2[static_cast<size_t>(r0) * (P2 + 1) + i];
 
   // partial window of this workgroup: [w3max][nout][w2max][w1max]
   const size_t plane = static_cast<size_t>(a.w1max) * a.w2max;
@@ -156,7 +170,7 @@ __global__ __launch_bounds__(kThreads) void adj_stream(AdjStreamArgs<T> a) {
 
   const int rb = r0 + wave * a.rows_per_wave;
   const int re = min(rb + a.rows_per_wave, r1);
-  T* Sw = S + static_cast<size_t>(wave) * a.w2wave * NC * kLanes;
+  T* Sw = S + static_cast<size_t>(wave) * a.w2wave * NC * LS;
   int cur3 = a.tab3 ? a.span3[j3a] : 0;
   for (int j3 = j3a; j3 < j3b; ++j3) {
     const int s3 = a.tab3 ? a.span3[j3] : 0;
@@ -168,7 +182,7 @@ __global__ __launch_bounds__(kThreads) void adj_stream(AdjStreamArgs<T> a) {
 
     // ---- A: stream the wave's rows, sliding window over k2 ----
     if (rb < re) {
-      int cur = a.span2[rb];
+      int cur = RS[rb - r0];
       const int w0 = cur - P2;
       T U[P2 + 1][NC];
 #pragma unroll
@@ -189,11 +203,11 @@ __global__ __launch_bounds__(kThreads) void adj_stream(AdjStreamArgs<T> a) {
         for (int r = 0; r < kRowBatch; ++r) {
           const int j2 = jb + r;
           if (j2 >= re) break;
-          const int s2 = a.span2[j2];
+          const int s2 = RS[j2 - r0];
           while (cur < s2) {
             const int row = cur - P2 - w0;
 #pragma unroll
-            for (int c = 0; c < NC; ++c) Sw[(row * NC + c) * kLanes + lane] = U[0][c];
+            for (int c = 0; c < NC; ++c) Sw[(row * NC + c) * LS + lane] = U[0][c];
 #pragma unroll
             for (int q = 0; q < P2; ++q)
 #pragma unroll
@@ -202,7 +216,7 @@ __global__ __launch_bounds__(kThreads) void adj_stream(AdjStreamArgs<T> a) {
             for (int c = 0; c < NC; ++c) U[P2][c] = T(0);
             ++cur;
           }
-          const T* b2p = a.tab2 + static_cast<size_t>(j2) * (P2 + 1);
+          const T* b2p = RT + static_cast<size_t>(j2 - r0) * (P2 + 1);
           T bb[P2 + 1];
 #pragma unroll
           for (int q = 0; q <= P2; ++q) bb[q] = b2p[q];
@@ -216,7 +230,7 @@ __global__ __launch_bounds__(kThreads) void adj_stream(AdjStreamArgs<T> a) {
       for (int q = 0; q <= P2; ++q) {
         const int row = cur - P2 + q - w0;
 #pragma unroll
-        for (int c = 0; c < NC; ++c) Sw[(row * NC + c) * kLanes + lane] = U[q][c];
+        for (int c = 0; c < NC; ++c) Sw[(row * NC + c) * LS + lane] = U[q][c];
       }
       if (lane == 0) {
         wlo[wave] = w0;
@@ -235,13 +249,13 @@ __global__ __launch_bounds__(kThreads) void adj_stream(AdjStreamArgs<T> a) {
       const int q2 = rest % W2;
       const int c = rest / W2;
       const int k2 = lo2 + q2;
-      const T* bt = BlT + q1 * kLanes;
+      const T* bt = BlT + q1 * LS;
       const int l0 = llo[q1], l1 = lhi[q1];
       T sum = T(0);
 #pragma unroll
       for (int w = 0; w < kWaves; ++w) {
         if (k2 < wlo[w] || k2 > whi[w]) continue;
-        const T* sw = S + ((static_cast<size_t>(w) * a.w2wave + (k2 - wlo[w])) * NC + c) * kLanes;
+        const T* sw = S + ((static_cast<size_t>(w) * a.w2wave + (k2 - wlo[w])) * NC + c) * LS;
         sum += dot_lanes(bt, sw, l0, l1);
       }
       if (a.tab3) {
@@ -262,15 +276,16 @@ template <typename T>
 struct AdjGatherArgs {
   const T* part;
   T* out;
-  const int32_t* tile_of_k1;
-  const int32_t* chunk_of_k2;
-  const int32_t* z_of_k3;
-  const int32_t* span1;
-  const int32_t* span2;
-  const int32_t* span3;
-  int n1, n2, nin;
-  int p1, p2, p3;
-  int nout, w1max, w2max, w3max, ntx, nty, rows_chunk, j3_per_block;
+  // per dimension d and control point k: the (block, offset) pairs of the
+  // workgroup windows that hold k, as a list [ptr[k], ptr[k+1])
+  const int32_t* ptr1;
+  const int2* ent1;
+  const int32_t* ptr2;
+  const int2* ent2;
+  const int32_t* ptr3;
+  const int2* ent3;
+  int n1, n2;
+  int nout, w1max, w2max, w3max, ntx, nty;
   int accumulate;
   long long ncp;
 };
@@ -285,24 +300,20 @@ __global__ __launch_bounds__(256) void adj_gather(AdjGatherArgs<T> a) {
   r /= a.n1;
   const int k2 = static_cast<int>(r % a.n2);
   const int k3 = static_cast<int>(r / a.n2);
-  const int ta = a.tile_of_k1[2 * k1], tb = a.tile_of_k1[2 * k1 + 1];
-  const int ca = a.chunk_of_k2[2 * k2], cb = a.chunk_of_k2[2 * k2 + 1];
-  int za = 0, zb = 0;
-  if (a.nin == 3) {
-    za = a.z_of_k3[2 * k3];
-    zb = a.z_of_k3[2 * k3 + 1];
-  }
   const size_t plane = static_cast<size_t>(a.w1max) * a.w2max;
+  const int e1a = a.ptr1[k1], e1b = a.ptr1[k1 + 1];
+  const int e2a = a.ptr2[k2], e2b = a.ptr2[k2 + 1];
+  const int e3a = a.ptr3[k3], e3b = a.ptr3[k3 + 1];
   T sum = T(0);
-  for (int bz = za; bz <= zb; ++bz) {
-    const int lo3 = a.nin == 3 ? a.span3[bz * a.j3_per_block] - a.p3 : 0;
-    for (int by = ca; by <= cb; ++by) {
-      const int lo2 = a.span2[by * a.rows_chunk] - a.p2;
-      for (int bx = ta; bx <= tb; ++bx) {
-        const int lo1 = a.span1[bx * kLanes] - a.p1;
-        const size_t unit = (static_cast<size_t>(bz) * a.nty + by) * a.ntx + bx;
-        sum += a.part[((unit * a.w3max + (k3 - lo3)) * a.nout + c) * plane +
-                      static_cast<size_t>(k2 - lo2) * a.w1max + (k1 - lo1)];
+  for (int e3 = e3a; e3 < e3b; ++e3) {
+    const int2 z = a.ent3[e3];
+    for (int e2 = e2a; e2 < e2b; ++e2) {
+      const int2 y = a.ent2[e2];
+      const size_t row = ((static_cast<size_t>(z.x) * a.nty + y.x) * a.ntx) * a.w3max;
+      for (int e1 = e1a; e1 < e1b; ++e1) {
+        const int2 x = a.ent1[e1];
+        const size_t unit_row = row + static_cast<size_t>(x.x) * a.w3max + z.y;
+        sum += a.part[(unit_row * a.nout + c) * plane + static_cast<size_t>(y.y) * a.w1max + x.y];
       }
     }
   }
@@ -416,10 +427,13 @@ AdjKernel<T> pick_adj(int p2, int nc) {
 size_t adjoint_stream_lds(const Plan& pl, int nc, size_t elem) {
   const Tiling& t = pl.adj;
   const int p3 = pl.nin == 3 ? pl.dims[2].p : 0;
-  return static_cast<size_t>(kWaves) * t.w2wave_max * nc * kLanes * elem +
-         static_cast<size_t>(t.w1max) * kLanes * elem +
+  const size_t ls = kLanes + 16 / elem;
+  const size_t chunk = static_cast<size_t>(t.chunk_rows());
+  return static_cast<size_t>(kWaves) * t.w2wave_max * nc * ls * elem +
+         static_cast<size_t>(t.w1max) * ls * elem +
          static_cast<size_t>(p3 + 1) * t.w1max * t.w2max * nc * elem +
-         (2 * kWaves + 2 * t.w1max) * sizeof(int);
+         chunk * (pl.dims[1].p + 1) * elem +
+         (chunk + 2 * kWaves + 2 * t.w1max) * sizeof(int);
 }
 
 namespace {
@@ -486,28 +500,20 @@ hipError_t adjoint_stream(const Plan& pl, const int32_t* orders, const T* E,
   AdjGatherArgs<T> g{};
   g.part = static_cast<const T*>(pl.work);
   g.out = Pbar;
-  g.tile_of_k1 = tl.tile_of_k1;
-  g.chunk_of_k2 = tl.chunk_of_k2;
-  g.z_of_k3 = tl.z_of_k3;
-  g.span1 = d1.span;
-  g.span2 = d2.span;
+  g.ptr1 = tl.gptr[0];
+  g.ent1 = reinterpret_cast<const int2*>(tl.gent[0]);
+  g.ptr2 = tl.gptr[1];
+  g.ent2 = reinterpret_cast<const int2*>(tl.gent[1]);
+  g.ptr3 = tl.gptr[2];
+  g.ent3 = reinterpret_cast<const int2*>(tl.gent[2]);
   g.n1 = d1.n;
   g.n2 = d2.n;
-  g.nin = nin;
-  g.p1 = d1.p;
-  g.p2 = d2.p;
-  if (nin == 3) {
-    g.span3 = pl.dims[2].span;
-    g.p3 = pl.dims[2].p;
-  }
   g.nout = pl.nout;
   g.w1max = tl.w1max;
   g.w2max = tl.w2max;
   g.w3max = tl.w3max;
   g.ntx = tl.n_tiles1;
   g.nty = tl.n_chunks2;
-  g.rows_chunk = tl.chunk_rows();
-  g.j3_per_block = tl.j3_per_block;
   g.accumulate = accumulate;
   g.ncp = 1;
   for (int d = 0; d < nin; ++d) g.ncp *= pl.dims[d].n;

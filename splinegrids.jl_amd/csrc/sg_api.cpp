@@ -52,16 +52,18 @@ void free_plan(Plan* pl) {
 
 // Band windows of a tiling: tiles of kLanes samples along dimension 1,
 // chunks of kWaves * rows_per_wave rows along dimension 2.
-void compute_tiling(const Plan& pl, Tiling& t, int rows_per_wave) {
+void compute_tiling(const Plan& pl, Tiling& t, int rows_per_wave, int wx = 1) {
   const DimDev& d1 = pl.dims[0];
   const DimDev& d2 = pl.dims[1];
   const std::vector<int32_t>& s1 = d1.h_span;
   const std::vector<int32_t>& s2 = d2.h_span;
   t.rows_per_wave = rows_per_wave;
-  t.n_tiles1 = (d1.m + kLanes - 1) / kLanes;
+  t.wx = wx;
+  const int tw = t.tile_width();
+  t.n_tiles1 = (d1.m + tw - 1) / tw;
   t.w1max = 0;
   for (int i = 0; i < t.n_tiles1; ++i) {
-    const int a = i * kLanes, b = std::min(a + kLanes, d1.m) - 1;
+    const int a = i * tw, b = std::min(a + tw, d1.m) - 1;
     t.w1max = std::max(t.w1max, s1[b] - s1[a] + d1.p + 1);
   }
   const int chunk = t.chunk_rows();
@@ -71,7 +73,7 @@ void compute_tiling(const Plan& pl, Tiling& t, int rows_per_wave) {
   for (int c = 0; c < t.n_chunks2; ++c) {
     const int a = c * chunk, b = std::min(a + chunk, d2.m) - 1;
     t.w2max = std::max(t.w2max, s2[b] - s2[a] + d2.p + 1);
-    for (int w = 0; w < kWaves; ++w) {
+    for (int w = 0; w < kWaves / wx; ++w) {
       const int ra = a + w * rows_per_wave;
       const int rb = std::min(ra + rows_per_wave, b + 1) - 1;
       if (ra > rb) continue;
@@ -80,9 +82,12 @@ void compute_tiling(const Plan& pl, Tiling& t, int rows_per_wave) {
   }
 }
 
-size_t forward_lds_worst(const Plan& pl) {
+size_t forward_lds_worst(const Plan& pl, const Tiling& t) {
   const int nc = std::min(pl.nout, 4) + (pl.nurbs ? 1 : 0);
-  return static_cast<size_t>(nc) * pl.fwd.w1max * pl.fwd.w2max * sizeof(double);
+  const size_t p2 = static_cast<size_t>(pl.dims[1].p);
+  return (static_cast<size_t>(nc) * t.w1max * t.w2max +
+          static_cast<size_t>(SG_MAX_FIELDS) * t.chunk_rows() * (p2 + 1)) * sizeof(double) +
+         static_cast<size_t>(t.chunk_rows()) * sizeof(int);
 }
 
 size_t adjoint_lds_min(const Plan& pl) {  // one channel, FP64
@@ -102,7 +107,7 @@ void compute_ztiling(const Plan& pl, Tiling& t, int j3_per_block) {
   const DimDev& d3 = pl.dims[2];
   if (j3_per_block <= 0) {
     const long long xy = static_cast<long long>(t.n_tiles1) * t.n_chunks2;
-    const long long target = 2048;
+    const long long target = 1024;
     j3_per_block = static_cast<int>(std::max<long long>(1, (xy * d3.m) / target));
   }
   t.j3_per_block = std::min(j3_per_block, d3.m);
@@ -344,15 +349,32 @@ int build_plan(const sg_grid_desc* desc, hipStream_t s, bool want_adjoint, Plan*
   ok = ok && pl->dims[0].p <= kMaxStreamDegree && pl->dims[1].p <= kMaxStreamDegree;
   pl->nt_stores = env_int("SG_NT_STORES", 0) ? 1 : 0;
   if (ok) {
-    int rows = env_int("SG_FWD_ROWS", 32);
-    compute_tiling(*pl, pl->fwd, rows);
-    while (forward_lds_worst(*pl) > static_cast<size_t>(kMaxLds) && rows > 1) {
-      rows /= 2;
-      compute_tiling(*pl, pl->fwd, rows);
-    }
-    ok = forward_lds_worst(*pl) <= static_cast<size_t>(kMaxLds);
     const char* xm = std::getenv("SG_XCD_MAP");
-    pl->fwd.xcd_map = (xm && *xm) ? std::atoi(xm) : default_xcd_map(*pl);
+    const int xcd = (xm && *xm) ? std::atoi(xm) : default_xcd_map(*pl);
+    // forward: waves side by side along dimension 1 when it is long enough
+    int wx = env_int("SG_FWD_WX", pl->dims[0].m >= 1024 ? 4 : 1);
+    if (wx != 1 && wx != 2 && wx != 4) wx = 1;
+    int rows = env_int("SG_FWD_ROWS", 32);
+    compute_tiling(*pl, pl->fwd, rows, wx);
+    while (forward_lds_worst(*pl, pl->fwd) > static_cast<size_t>(kMaxLds) &&
+           (rows > 1 || wx > 1)) {
+      if (wx > 1)
+        wx /= 2;
+      else
+        rows /= 2;
+      compute_tiling(*pl, pl->fwd, rows, wx);
+    }
+    pl->fwd.xcd_map = xcd;
+    // MFMA tiers: 64-sample tiles, each wave 16 samples of dimension 1
+    int mrows = 32;
+    compute_tiling(*pl, pl->mma, mrows, 1);
+    while (forward_lds_worst(*pl, pl->mma) > static_cast<size_t>(kMaxLds) && mrows > 4) {
+      mrows /= 2;
+      compute_tiling(*pl, pl->mma, mrows, 1);
+    }
+    if (forward_lds_worst(*pl, pl->mma) > static_cast<size_t>(kMaxLds)) pl->mma = Tiling();
+    pl->mma.xcd_map = xcd;
+    ok = forward_lds_worst(*pl, pl->fwd) <= static_cast<size_t>(kMaxLds);
   }
   if (ok && !pl->nurbs && want_adjoint) {
     int rows = env_int("SG_ADJ_ROWS", 16);
@@ -370,58 +392,51 @@ int build_plan(const sg_grid_desc* desc, hipStream_t s, bool want_adjoint, Plan*
   }
   pl->stream_ok = ok;
   if (ok && pl->adj.n_tiles1 > 0) {
-    const DimDev& d1 = pl->dims[0];
-    const DimDev& d2 = pl->dims[1];
     Tiling& tl = pl->adj;
-    std::vector<int32_t> t1(2 * d1.n), c2(2 * d2.n);
-    for (int k = 0; k < d1.n; ++k) {
-      t1[2 * k] = INT_MAX;
-      t1[2 * k + 1] = -1;
-    }
-    for (int k = 0; k < d2.n; ++k) {
-      c2[2 * k] = INT_MAX;
-      c2[2 * k + 1] = -1;
-    }
-    for (int t = 0; t < tl.n_tiles1; ++t) {
-      const int a = t * kLanes, b = std::min(a + kLanes, d1.m) - 1;
-      for (int k = d1.h_span[a] - d1.p; k <= d1.h_span[b]; ++k) {
-        t1[2 * k] = std::min(t1[2 * k], t);
-        t1[2 * k + 1] = std::max(t1[2 * k + 1], t);
-      }
-    }
-    const int chunk = tl.chunk_rows();
-    for (int c = 0; c < tl.n_chunks2; ++c) {
-      const int a = c * chunk, b = std::min(a + chunk, d2.m) - 1;
-      for (int k = d2.h_span[a] - d2.p; k <= d2.h_span[b]; ++k) {
-        c2[2 * k] = std::min(c2[2 * k], c);
-        c2[2 * k + 1] = std::max(c2[2 * k + 1], c);
-      }
-    }
-    std::vector<int32_t> z3;
-    if (pl->nin == 3) {
-      const DimDev& d3 = pl->dims[2];
-      z3.assign(2 * d3.n, 0);
-      for (int k = 0; k < d3.n; ++k) {
-        z3[2 * k] = INT_MAX;
-        z3[2 * k + 1] = -1;
-      }
-      for (int z = 0; z < tl.n_z; ++z) {
-        const int a = z * tl.j3_per_block, b = std::min(a + tl.j3_per_block, d3.m) - 1;
-        for (int k = d3.h_span[a] - d3.p; k <= d3.h_span[b]; ++k) {
-          z3[2 * k] = std::min(z3[2 * k], z);
-          z3[2 * k + 1] = std::max(z3[2 * k + 1], z);
+    // window [lo, hi] of every block along dimension d, then per-k lists
+    auto lists = [&](int d, int nblocks, int block_len, int m, int n, int p,
+                     const std::vector<int32_t>& span, std::vector<int32_t>& ptr,
+                     std::vector<int32_t>& ent) {
+      std::vector<std::vector<int32_t>> per(n);
+      for (int b = 0; b < nblocks; ++b) {
+        const int a = b * block_len, e = std::min(a + block_len, m) - 1;
+        const int lo = span[a] - p;
+        for (int k = lo; k <= span[e]; ++k) {
+          per[k].push_back(b);
+          per[k].push_back(k - lo);
         }
       }
+      ptr.assign(n + 1, 0);
+      ent.clear();
+      for (int k = 0; k < n; ++k) {
+        ent.insert(ent.end(), per[k].begin(), per[k].end());
+        ptr[k + 1] = static_cast<int32_t>(ent.size() / 2);
+      }
+      (void)d;
+    };
+    std::vector<int32_t> ptr[3], ent[3];
+    const DimDev& d1 = pl->dims[0];
+    const DimDev& d2 = pl->dims[1];
+    lists(0, tl.n_tiles1, tl.tile_width(), d1.m, d1.n, d1.p, d1.h_span, ptr[0], ent[0]);
+    lists(1, tl.n_chunks2, tl.chunk_rows(), d2.m, d2.n, d2.p, d2.h_span, ptr[1], ent[1]);
+    if (pl->nin == 3) {
+      const DimDev& d3 = pl->dims[2];
+      lists(2, tl.n_z, tl.j3_per_block, d3.m, d3.n, d3.p, d3.h_span, ptr[2], ent[2]);
+    } else {
+      ptr[2] = {0, 1};
+      ent[2] = {0, 0};
     }
-    hipError_t e = dalloc(*pl, &tl.tile_of_k1, t1.size());
-    if (e == hipSuccess) e = dalloc(*pl, &tl.chunk_of_k2, c2.size());
-    if (e == hipSuccess && !z3.empty()) e = dalloc(*pl, &tl.z_of_k3, z3.size());
-    if (e == hipSuccess && !z3.empty())
-      e = hipMemcpy(tl.z_of_k3, z3.data(), z3.size() * sizeof(int32_t), hipMemcpyHostToDevice);
-    if (e == hipSuccess)
-      e = hipMemcpy(tl.tile_of_k1, t1.data(), t1.size() * sizeof(int32_t), hipMemcpyHostToDevice);
-    if (e == hipSuccess)
-      e = hipMemcpy(tl.chunk_of_k2, c2.data(), c2.size() * sizeof(int32_t), hipMemcpyHostToDevice);
+    hipError_t e = hipSuccess;
+    for (int d = 0; d < 3 && e == hipSuccess; ++d) {
+      e = dalloc(*pl, &tl.gptr[d], ptr[d].size());
+      if (e == hipSuccess) e = dalloc(*pl, &tl.gent[d], ent[d].size());
+      if (e == hipSuccess)
+        e = hipMemcpy(tl.gptr[d], ptr[d].data(), ptr[d].size() * sizeof(int32_t),
+                      hipMemcpyHostToDevice);
+      if (e == hipSuccess && !ent[d].empty())
+        e = hipMemcpy(tl.gent[d], ent[d].data(), ent[d].size() * sizeof(int32_t),
+                      hipMemcpyHostToDevice);
+    }
     pl->work_bytes = adjoint_work_bytes(*pl);
     unsigned char* w = nullptr;
     if (e == hipSuccess) e = dalloc(*pl, &w, pl->work_bytes);

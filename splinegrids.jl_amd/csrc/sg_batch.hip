@@ -1,0 +1,551 @@
+// sg_batch.hip -- many independent grids in one persistent launch.
+//
+// A batch holds one plan (FP64 tables, spans) per grid and a single list of
+// tiles over all grids -- the same tiles as fwd_stream: 64 samples of
+// dimension 1 x a chunk of rows of dimension 2 x one sample of dimension 3
+// -- sorted largest first.  A persistent kernel with a few workgroups per CU
+// takes tiles from a device work queue (one atomic counter in global memory,
+// reset by a one-thread kernel before each launch) until the queue is empty,
+// so uneven grids balance across the chip.  Degrees, channel counts, NURBS
+// weights and sizes are per tile at run time; the p2+1 register window is
+// selected by a switch over p2.
+#include <algorithm>
+#include <numeric>
+#include <vector>
+
+#include "sg_tiles.hpp"
+
+namespace sg {
+
+struct BGrid {
+  const float* P;
+  const float* W;
+  float* out;
+  const float* tab1;
+  const float* tab2;
+  const float* tab3;
+  const int32_t* span1;
+  const int32_t* span2;
+  const int32_t* span3;
+  int nin, nout, nurbs;
+  int n1, n2, p1, p2, p3;
+  int m1, m2, m3;
+  int rows_per_wave;
+  int pad;
+  long long ncp, nsamp;
+};
+
+struct BTile {
+  int grid, bx, by, j3;
+};
+
+namespace {
+
+constexpr int kBC = 4;  // value channels per pass; the weight is slab 4
+
+__global__ void batch_reset(int* counter) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *counter = 0;
+}
+
+template <int P2>
+__device__ void batch_tile(const BGrid& g, const BTile& t, float* smem, int gcap, int lane,
+                           int wave) {
+  constexpr int PM = kMaxStreamDegree;
+  const int chunk_cap = g.rows_per_wave * kWaves;
+  const int j1_0 = t.bx * kLanes;
+  const int j1_last = min(j1_0 + kLanes - 1, g.m1 - 1);
+  const int r0 = t.by * chunk_cap, r1 = min(r0 + chunk_cap, g.m2);
+  const int p1 = g.p1;
+  const int lo1 = g.span1[j1_0] - p1;
+  const int W1 = g.span1[j1_last] - lo1 + 1;
+  const int lo2 = g.span2[r0] - P2;
+  const int W2 = g.span2[r1 - 1] - lo2 + 1;
+  const int cs = W1 * W2;
+  float* G = smem;                                        // [5][gcap]
+  float* RT = smem + static_cast<size_t>(kBC + 1) * gcap;  // [chunk][P2+1]
+  int* RS = reinterpret_cast<int*>(RT + static_cast<size_t>(chunk_cap) * (P2 + 1));
+  const bool nurbs = g.nurbs != 0;
+  const int j3 = t.j3;
+
+  for (int i = threadIdx.x; i < r1 - r0; i += kThreads) RS[i] = g.span2[r0 + i];
+  for (int i = threadIdx.x; i < (r1 - r0) * (P2 + 1); i += kThreads)
+    RT[i] = g.tab2[static_cast<size_t>(r0) * (P2 + 1) + i];
+
+  const int rb = r0 + wave * g.rows_per_wave;
+  const int re = min(rb + g.rows_per_wave, r1);
+  const bool active = (j1_0 + lane) < g.m1;
+  const int j1 = active ? j1_0 + lane : g.m1 - 1;
+  const int q1b = g.span1[j1] - p1 - lo1;
+  float b1[PM + 1];
+#pragma unroll
+  for (int q = 0; q <= PM; ++q)
+    b1[q] = (q <= p1) ? g.tab1[static_cast<size_t>(j1) * (p1 + 1) + min(q, p1)] : 0.f;
+
+  for (int c0 = 0; c0 < g.nout; c0 += kBC) {
+    const int nc = min(kBC, g.nout - c0);
+    // ---- stage 1: band window, contracted along dimension 3 ----
+    for (int idx = threadIdx.x; idx < cs; idx += kThreads) {
+      const int q2 = idx / W1, q1 = idx - q2 * W1;
+      const long long cpi =
+          static_cast<long long>(lo1 + q1) + static_cast<long long>(g.n1) * (lo2 + q2);
+      float acc[kBC] = {0.f, 0.f, 0.f, 0.f};
+      float accw = 0.f;
+      if (g.nin == 2) {
+        float w = 1.f;
+        if (nurbs) w = g.W[cpi];
+        accw = w;
+#pragma unroll
+        for (int c = 0; c < kBC; ++c)
+          if (c < nc) acc[c] = w * g.P[cpi + g.ncp * (c0 + c)];
+      } else {
+        const int s3 = g.span3[j3];
+        const float* b3 = g.tab3 + static_cast<size_t>(j3) * (g.p3 + 1);
+        const long long plane = static_cast<long long>(g.n1) * g.n2;
+        for (int d = 0; d <= g.p3; ++d) {
+          const long long off = cpi + plane * (s3 - g.p3 + d);
+          float bw = b3[d];
+          if (nurbs) bw *= g.W[off];
+          accw += bw;
+#pragma unroll
+          for (int c = 0; c < kBC; ++c)
+            if (c < nc) acc[c] += bw * g.P[off + g.ncp * (c0 + c)];
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < kBC; ++c)
+        if (c < nc) G[c * gcap + idx] = acc[c];
+      if (nurbs) G[kBC * gcap + idx] = accw;
+    }
+    __syncthreads();
+
+    // ---- stage 2: rows of this wave, sliding window over k2 ----
+    if (rb < re) {
+      float V[P2 + 1][kBC + 1];
+      auto column = [&](int k2, float (&v)[kBC + 1]) {
+        const float* gr = G + (k2 - lo2) * W1 + q1b;
+#pragma unroll
+        for (int ch = 0; ch <= kBC; ++ch) {
+          float s = 0.f;
+          if (ch < nc || (ch == kBC && nurbs)) {
+#pragma unroll
+            for (int q = 0; q <= PM; ++q)
+              if (q <= p1) s += b1[q] * gr[ch * gcap + min(q, p1)];
+          }
+          v[ch] = s;
+        }
+      };
+      int cur = RS[rb - r0];
+#pragma unroll
+      for (int q = 0; q <= P2; ++q) column(cur - P2 + q, V[q]);
+      const long long obase = static_cast<long long>(j1) +
+                              static_cast<long long>(g.m1) * g.m2 * j3 + g.nsamp * c0;
+      for (int j2 = rb; j2 < re; ++j2) {
+        const int s2 = RS[j2 - r0];
+        if (s2 != cur) {
+          if (s2 - cur > P2) {
+            cur = s2;
+#pragma unroll
+            for (int q = 0; q <= P2; ++q) column(cur - P2 + q, V[q]);
+          } else {
+            while (cur < s2) {
+#pragma unroll
+              for (int q = 0; q < P2; ++q)
+#pragma unroll
+                for (int ch = 0; ch <= kBC; ++ch) V[q][ch] = V[q + 1][ch];
+              ++cur;
+              column(cur, V[P2]);
+            }
+          }
+        }
+        const float* bb = RT + static_cast<size_t>(j2 - r0) * (P2 + 1);
+        float num[kBC + 1];
+#pragma unroll
+        for (int ch = 0; ch <= kBC; ++ch) {
+          float s = 0.f;
+#pragma unroll
+          for (int q = 0; q <= P2; ++q) s += bb[q] * V[q][ch];
+          num[ch] = s;
+        }
+        if (active) {
+          const long long o = obase + static_cast<long long>(g.m1) * j2;
+#pragma unroll
+          for (int c = 0; c < kBC; ++c)
+            if (c < nc) g.out[o + g.nsamp * c] = nurbs ? num[c] / num[kBC] : num[c];
+        }
+      }
+    }
+    __syncthreads();  // G is rewritten by the next channel block / tile
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void batch_persistent(const BGrid* __restrict__ grids,
+                                                             const BTile* __restrict__ tiles,
+                                                             int n_tiles, int gcap,
+                                                             int* counter) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  float* smem = reinterpret_cast<float*>(smem_raw);
+  __shared__ int s_tile;
+  const int lane = threadIdx.x & (kLanes - 1);
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kLanes);
+  for (;;) {
+    if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1);
+    __syncthreads();
+    const int ti = s_tile;
+    if (ti >= n_tiles) break;  // every thread read the same value: uniform exit
+    const BTile t = tiles[ti];
+    const BGrid g = grids[t.grid];
+    switch (g.p2) {
+      case 0: batch_tile<0>(g, t, smem, gcap, lane, wave); break;
+      case 1: batch_tile<1>(g, t, smem, gcap, lane, wave); break;
+      case 2: batch_tile<2>(g, t, smem, gcap, lane, wave); break;
+      case 3: batch_tile<3>(g, t, smem, gcap, lane, wave); break;
+      case 4: batch_tile<4>(g, t, smem, gcap, lane, wave); break;
+      default: batch_tile<5>(g, t, smem, gcap, lane, wave); break;
+    }
+    // batch_tile ends with a barrier, so s_tile may be overwritten now
+  }
+}
+
+}  // namespace
+
+}  // namespace sg
+
+using namespace sg;
+
+struct sg_batch {
+  std::vector<Plan*> plans;       // one per grid
+  std::vector<int> persistent;    // grids the persistent kernel evaluates
+  std::vector<int> separate;      // grids evaluated with their own launches
+  std::vector<BGrid> hgrids;
+  BGrid* dgrids = nullptr;
+  BTile* dtiles = nullptr;
+  int* counter = nullptr;
+  int n_tiles = 0;
+  int gcap = 0;                   // LDS floats per G slab
+  size_t lds = 0;
+  int rows_per_wave = 16;
+  int blocks_per_cu = 4;
+  int n_cu = 0;
+  int device = 0;
+  long long n_samples = 0;
+  std::vector<const void*> P, W;
+  std::vector<void*> out;
+};
+
+namespace {
+
+void free_batch(sg_batch* b) {
+  if (!b) return;
+  for (Plan* p : b->plans) destroy_plan(p);
+  if (b->dgrids) (void)hipFree(b->dgrids);
+  if (b->dtiles) (void)hipFree(b->dtiles);
+  if (b->counter) (void)hipFree(b->counter);
+  delete b;
+}
+
+// Tiles of every persistent grid for a rows-per-wave choice; largest first.
+int build_tiles(sg_batch* b, int rows_per_wave) {
+  std::vector<BTile> tiles;
+  std::vector<long long> cost;
+  int gcap = 1;
+  size_t row_words = 0;
+  for (int gi : b->persistent) {
+    const Plan& pl = *b->plans[gi];
+    Tiling t;
+    const DimDev& d1 = pl.dims[0];
+    const DimDev& d2 = pl.dims[1];
+    t.rows_per_wave = rows_per_wave;
+    t.n_tiles1 = (d1.m + kLanes - 1) / kLanes;
+    const int chunk = rows_per_wave * kWaves;
+    t.n_chunks2 = (d2.m + chunk - 1) / chunk;
+    int w1 = 0, w2 = 0;
+    for (int i = 0; i < t.n_tiles1; ++i) {
+      const int a = i * kLanes, e = std::min(a + kLanes, d1.m) - 1;
+      w1 = std::max(w1, d1.h_span[e] - d1.h_span[a] + d1.p + 1);
+    }
+    for (int c = 0; c < t.n_chunks2; ++c) {
+      const int a = c * chunk, e = std::min(a + chunk, d2.m) - 1;
+      w2 = std::max(w2, d2.h_span[e] - d2.h_span[a] + d2.p + 1);
+    }
+    gcap = std::max(gcap, w1 * w2);
+    row_words = std::max(row_words, static_cast<size_t>(chunk) * (d2.p + 2));
+    const int m3 = pl.nin == 3 ? pl.dims[2].m : 1;
+    for (int z = 0; z < m3; ++z)
+      for (int y = 0; y < t.n_chunks2; ++y)
+        for (int x = 0; x < t.n_tiles1; ++x) {
+          tiles.push_back(BTile{gi, x, y, z});
+          const long long lanes = std::min(kLanes, d1.m - x * kLanes);
+          const long long rows = std::min(chunk, d2.m - y * chunk);
+          cost.push_back(lanes * rows * pl.nout + 64 * w1 * w2);
+        }
+    b->hgrids[gi].rows_per_wave = rows_per_wave;
+  }
+  std::vector<size_t> order(tiles.size());
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](size_t a, size_t c) { return cost[a] > cost[c]; });
+  std::vector<BTile> sorted(tiles.size());
+  for (size_t i = 0; i < order.size(); ++i) sorted[i] = tiles[order[i]];
+  const size_t lds = (static_cast<size_t>(kBC + 1) * gcap + row_words) * sizeof(float);
+  if (lds > static_cast<size_t>(kMaxLds)) return fail(SG_ERR_UNSUPPORTED, "batch tiles exceed LDS");
+  if (b->dtiles) (void)hipFree(b->dtiles);
+  b->dtiles = nullptr;
+  hipError_t e = hipSuccess;
+  if (!sorted.empty()) {
+    e = hipMalloc(&b->dtiles, sorted.size() * sizeof(BTile));
+    if (e == hipSuccess)
+      e = hipMemcpy(b->dtiles, sorted.data(), sorted.size() * sizeof(BTile), hipMemcpyHostToDevice);
+  }
+  if (e == hipSuccess && !b->hgrids.empty())
+    e = hipMemcpy(b->dgrids, b->hgrids.data(), b->hgrids.size() * sizeof(BGrid),
+                  hipMemcpyHostToDevice);
+  if (e != hipSuccess) return hip_fail(e, "batch tile upload");
+  b->n_tiles = static_cast<int>(sorted.size());
+  b->gcap = gcap;
+  b->lds = lds;
+  b->rows_per_wave = rows_per_wave;
+  return SG_OK;
+}
+
+hipError_t launch_batch(sg_batch* b, hipStream_t s) {
+  hipLaunchKernelGGL(batch_reset, dim3(1), dim3(64), 0, s, b->counter);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || b->n_tiles == 0) return e;
+  const int blocks = std::max(1, std::min(b->n_tiles, b->blocks_per_cu * b->n_cu));
+  hipLaunchKernelGGL(batch_persistent, dim3(blocks), dim3(kThreads), b->lds, s, b->dgrids,
+                     b->dtiles, b->n_tiles, b->gcap, b->counter);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+extern "C" {
+
+int sg_batch_create(const sg_grid_desc* grids, int32_t n_grids, void* stream, sg_batch** out) {
+  if (!out) return fail(SG_ERR_INVALID, "batch output pointer is NULL");
+  *out = nullptr;
+  if (n_grids < 0 || (n_grids > 0 && !grids)) return fail(SG_ERR_INVALID, "bad grid list");
+  const int device = n_grids > 0 ? grids[0].device : 0;
+  for (int i = 0; i < n_grids; ++i) {
+    int rc = validate_desc(&grids[i]);
+    if (rc != SG_OK) return fail(rc, "grid " + std::to_string(i) + ": " + sg_last_error());
+    if (grids[i].device != device) return fail(SG_ERR_INVALID, "all grids must share a device");
+  }
+  int prev = 0;
+  (void)hipGetDevice(&prev);
+  if (prev != device) (void)hipSetDevice(device);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  sg_batch* b = new sg_batch();
+  b->device = device;
+  auto bail = [&](int rc) {
+    free_batch(b);
+    if (prev != device) (void)hipSetDevice(prev);
+    return rc;
+  };
+  hipDeviceProp_t prop;
+  hipError_t e = hipGetDeviceProperties(&prop, device);
+  if (e != hipSuccess) return bail(hip_fail(e, "hipGetDeviceProperties"));
+  b->n_cu = prop.multiProcessorCount;
+  b->plans.assign(n_grids, nullptr);
+  b->hgrids.assign(n_grids, BGrid{});
+  for (int i = 0; i < n_grids; ++i) {
+    int rc = build_plan(&grids[i], s, false, &b->plans[i]);
+    if (rc != SG_OK) return bail(rc);
+    const Plan& pl = *b->plans[i];
+    long long ns = 1;
+    for (const DimDev& d : pl.dims) ns *= d.m;
+    b->n_samples += ns;
+    const bool ok = pl.stream_ok && pl.dims[1].p <= kMaxStreamDegree;
+    (ok ? b->persistent : b->separate).push_back(i);
+    BGrid& g = b->hgrids[i];
+    g.nin = pl.nin;
+    g.nout = pl.nout;
+    g.nurbs = pl.nurbs;
+    const DimDev& d1 = pl.dims[0];
+    g.n1 = d1.n;
+    g.p1 = d1.p;
+    g.m1 = d1.m;
+    g.tab1 = d1.ktab32;
+    g.span1 = d1.span;
+    g.ncp = d1.n;
+    g.nsamp = d1.m;
+    if (pl.nin >= 2) {
+      const DimDev& d2 = pl.dims[1];
+      g.n2 = d2.n;
+      g.p2 = d2.p;
+      g.m2 = d2.m;
+      g.tab2 = d2.ktab32;
+      g.span2 = d2.span;
+      g.ncp *= d2.n;
+      g.nsamp *= d2.m;
+    }
+    g.m3 = 1;
+    if (pl.nin == 3) {
+      const DimDev& d3 = pl.dims[2];
+      g.p3 = d3.p;
+      g.m3 = d3.m;
+      g.tab3 = d3.ktab32;
+      g.span3 = d3.span;
+      g.ncp *= d3.n;
+      g.nsamp *= d3.m;
+    }
+  }
+  if (n_grids > 0) {
+    e = hipMalloc(&b->dgrids, n_grids * sizeof(BGrid));
+    if (e == hipSuccess) e = hipMalloc(&b->counter, sizeof(int));
+    if (e != hipSuccess) return bail(hip_fail(e, "batch allocation"));
+  }
+  const int rc = build_tiles(b, 16);
+  if (rc != SG_OK) return bail(rc);
+  b->P.assign(n_grids, nullptr);
+  b->W.assign(n_grids, nullptr);
+  b->out.assign(n_grids, nullptr);
+  if (prev != device) (void)hipSetDevice(prev);
+  *out = b;
+  return SG_OK;
+}
+
+int sg_batch_destroy(sg_batch* batch) {
+  if (!batch) return SG_OK;
+  int prev = 0;
+  (void)hipGetDevice(&prev);
+  if (prev != batch->device) (void)hipSetDevice(batch->device);
+  const int device = batch->device;
+  free_batch(batch);
+  if (prev != device) (void)hipSetDevice(prev);
+  return SG_OK;
+}
+
+int sg_batch_set_arrays(sg_batch* b, const void* const* control_points, const void* const* weights,
+                        void* const* evals) {
+  if (!b) return fail(SG_ERR_INVALID, "batch is NULL");
+  const int n = static_cast<int>(b->plans.size());
+  if (n > 0 && (!control_points || !evals)) return fail(SG_ERR_INVALID, "NULL array list");
+  for (int i = 0; i < n; ++i) {
+    const bool nurbs = b->plans[i]->nurbs != 0;
+    const void* w = weights ? weights[i] : nullptr;
+    if (nurbs != (w != nullptr))
+      return fail(SG_ERR_INVALID, "grid " + std::to_string(i) +
+                                      (nurbs ? ": NURBS grid needs weights" : ": B-spline grid takes no weights"));
+    if (b->hgrids[i].nsamp > 0 && (!control_points[i] || !evals[i]))
+      return fail(SG_ERR_INVALID, "grid " + std::to_string(i) + ": null device pointer");
+    b->P[i] = control_points[i];
+    b->W[i] = w;
+    b->out[i] = evals[i];
+    b->hgrids[i].P = static_cast<const float*>(control_points[i]);
+    // B-spline grids never read W; it still points at valid memory
+    b->hgrids[i].W = static_cast<const float*>(w ? w : control_points[i]);
+    b->hgrids[i].out = static_cast<float*>(evals[i]);
+  }
+  int prev = 0;
+  (void)hipGetDevice(&prev);
+  if (prev != b->device) (void)hipSetDevice(b->device);
+  hipError_t e = hipSuccess;
+  if (n > 0)
+    e = hipMemcpy(b->dgrids, b->hgrids.data(), n * sizeof(BGrid), hipMemcpyHostToDevice);
+  if (prev != b->device) (void)hipSetDevice(prev);
+  return e == hipSuccess ? SG_OK : hip_fail(e, "batch array upload");
+}
+
+int sg_batch_evaluate(sg_batch* b, int32_t tier, void* stream) {
+  if (!b) return fail(SG_ERR_INVALID, "batch is NULL");
+  if (tier != SG_TIER_NATIVE)
+    return fail(SG_ERR_UNSUPPORTED, "batches implement the native tier only");
+  for (size_t i = 0; i < b->out.size(); ++i)
+    if (!b->out[i] && b->hgrids[i].nsamp > 0)
+      return fail(SG_ERR_INVALID, "sg_batch_set_arrays was not called");
+  int prev = 0;
+  (void)hipGetDevice(&prev);
+  if (prev != b->device) (void)hipSetDevice(b->device);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipError_t e = launch_batch(b, s);
+  const int32_t zero[SG_MAX_NIN] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (size_t k = 0; e == hipSuccess && k < b->separate.size(); ++k) {
+    const int i = b->separate[k];
+    void* outs[1] = {b->out[i]};
+    e = launch_forward(*b->plans[i], 1, zero, b->P[i], b->W[i], outs, SG_F32, s);
+  }
+  if (prev != b->device) (void)hipSetDevice(prev);
+  return e == hipSuccess ? SG_OK : hip_fail(e, "batch launch");
+}
+
+int sg_batch_evaluate_each(sg_batch* b, void* stream) {
+  if (!b) return fail(SG_ERR_INVALID, "batch is NULL");
+  int prev = 0;
+  (void)hipGetDevice(&prev);
+  if (prev != b->device) (void)hipSetDevice(b->device);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int32_t zero[SG_MAX_NIN] = {0, 0, 0, 0, 0, 0, 0, 0};
+  hipError_t e = hipSuccess;
+  for (size_t i = 0; e == hipSuccess && i < b->plans.size(); ++i) {
+    if (!b->out[i] && b->hgrids[i].nsamp > 0) {
+      if (prev != b->device) (void)hipSetDevice(prev);
+      return fail(SG_ERR_INVALID, "sg_batch_set_arrays was not called");
+    }
+    void* outs[1] = {b->out[i]};
+    e = launch_forward(*b->plans[i], 1, zero, b->P[i], b->W[i], outs, SG_F32, s);
+  }
+  if (prev != b->device) (void)hipSetDevice(prev);
+  return e == hipSuccess ? SG_OK : hip_fail(e, "per-grid launch");
+}
+
+int sg_batch_autotune(sg_batch* b, int32_t reps, void* stream) {
+  if (!b) return fail(SG_ERR_INVALID, "batch is NULL");
+  for (size_t i = 0; i < b->out.size(); ++i)
+    if (!b->out[i] && b->hgrids[i].nsamp > 0)
+      return fail(SG_ERR_INVALID, "sg_batch_set_arrays was not called");
+  if (reps < 1) reps = 3;
+  int prev = 0;
+  (void)hipGetDevice(&prev);
+  if (prev != b->device) (void)hipSetDevice(b->device);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipEvent_t t0, t1;
+  (void)hipEventCreate(&t0);
+  (void)hipEventCreate(&t1);
+  float best = 1e30f;
+  int best_rows = b->rows_per_wave, best_bpc = b->blocks_per_cu;
+  int rc = SG_OK;
+  for (int rows : {8, 16, 32}) {
+    if ((rc = build_tiles(b, rows)) != SG_OK) continue;
+    for (int bpc : {2, 4, 8}) {
+      b->blocks_per_cu = bpc;
+      hipError_t e = launch_batch(b, s);  // warm-up
+      if (e == hipSuccess) e = hipEventRecord(t0, s);
+      for (int r = 0; r < reps && e == hipSuccess; ++r) e = launch_batch(b, s);
+      if (e == hipSuccess) e = hipEventRecord(t1, s);
+      if (e == hipSuccess) e = hipEventSynchronize(t1);
+      float ms = 0.f;
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, t0, t1);
+      if (e != hipSuccess) {
+        rc = hip_fail(e, "autotune launch");
+        break;
+      }
+      if (ms < best) {
+        best = ms;
+        best_rows = rows;
+        best_bpc = bpc;
+      }
+    }
+    if (rc != SG_OK && rc != SG_ERR_UNSUPPORTED) break;
+  }
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  b->blocks_per_cu = best_bpc;
+  const int rc2 = build_tiles(b, best_rows);
+  if (prev != b->device) (void)hipSetDevice(prev);
+  if (rc != SG_OK && rc != SG_ERR_UNSUPPORTED) return rc;
+  return rc2;
+}
+
+int sg_batch_info(const sg_batch* b, int64_t* n_tiles, int64_t* n_samples, int32_t* n_persistent,
+                  int32_t* rows_per_wave, int32_t* blocks_per_cu) {
+  if (!b) return fail(SG_ERR_INVALID, "batch is NULL");
+  if (n_tiles) *n_tiles = b->n_tiles;
+  if (n_samples) *n_samples = b->n_samples;
+  if (n_persistent) *n_persistent = static_cast<int32_t>(b->persistent.size());
+  if (rows_per_wave) *rows_per_wave = b->rows_per_wave;
+  if (blocks_per_cu) *blocks_per_cu = b->blocks_per_cu;
+  return SG_OK;
+}
+
+}  // extern "C"

@@ -42,9 +42,11 @@ struct StreamArgs {
   int c0;                        // first channel handled by this launch
   int nf;                        // fields actually stored (<= NF)
   int rows_per_wave;
+  int wx;                        // waves side by side along dimension 1
   int ntx, nty;                  // tiles along dimension 1, chunks along 2
   int xcd_map;                   // workgroup numbering mode
   int nt;                        // non-temporal stores of eval
+  int gmax;                      // LDS elements reserved per channel of G
   long long nsamp;               // elements per channel of eval
 };
 
@@ -57,25 +59,41 @@ __global__ __launch_bounds__(kThreads) void fwd_stream(StreamArgs<T> a) {
 
   const int lane = threadIdx.x & (kLanes - 1);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kLanes);
-  const FwdTile tile = fwd_tile(a.xcd_map, a.ntx, a.nty, a.rows_per_wave * kWaves, a.m1,
-                                a.m2, NIN);
+  const int wy = kWaves / a.wx;
+  const FwdTile tile = fwd_tile(a.xcd_map, a.ntx, a.nty, kLanes * a.wx, a.rows_per_wave * wy,
+                                a.m1, a.m2, NIN);
   const int j1_0 = tile.j1_0, r0 = tile.r0, r1 = tile.r1, j3 = tile.j3;
   const int lo1 = a.span1[j1_0] - a.p1;
   const int W1 = a.span1[tile.j1_last] - lo1 + 1;
   const int lo2 = a.span2[r0] - P2;
   const int W2 = a.span2[r1 - 1] - lo2 + 1;
   const int cs = W1 * W2;  // LDS elements per channel
+  // Row data of the chunk in LDS: spans and B2 of every field.  Reading them
+  // from LDS keeps vector-memory loads (which share vmcnt with the stores of
+  // eval) out of the row loop, so no row waits for earlier stores.
+  const int chunk = r1 - r0;
+  const int chunk_cap = a.rows_per_wave * (kWaves / a.wx);
+  T* RT = G + static_cast<size_t>(NCH) * a.gmax;                  // [NF][chunk][P2+1]
+  int* RS = reinterpret_cast<int*>(RT + static_cast<size_t>(NF) * chunk_cap * (P2 + 1));
 
   // ---- stage 1 ----
   stage1_band_window<T, NIN, NC, NURBS>(G, a.src, j3, lo1, W1, lo2, W2);
+  for (int i = threadIdx.x; i < chunk; i += kThreads) RS[i] = a.span2[r0 + i];
+  for (int i = threadIdx.x; i < NF * chunk * (P2 + 1); i += kThreads) {
+    const int f = i / (chunk * (P2 + 1)), rest = i - f * chunk * (P2 + 1);
+    RT[static_cast<size_t>(f) * chunk_cap * (P2 + 1) + rest] =
+        a.tab2[f][static_cast<size_t>(r0) * (P2 + 1) + rest];
+  }
   __syncthreads();
 
   // ---- stage 2 ----
-  const int rb = r0 + wave * a.rows_per_wave;
+  const int wave_x = wave % a.wx, wave_y = wave / a.wx;
+  const int rb = r0 + wave_y * a.rows_per_wave;
   const int re = min(rb + a.rows_per_wave, r1);
   if (rb >= re) return;  // no barrier follows
-  const bool active = (j1_0 + lane) < a.m1;
-  const int j1 = active ? j1_0 + lane : a.m1 - 1;
+  const int j1w = j1_0 + kLanes * wave_x + lane;
+  const bool active = j1w < a.m1;
+  const int j1 = active ? j1w : a.m1 - 1;
   const int p1 = a.p1;
   const int q1b = a.span1[j1] - p1 - lo1;
 
@@ -102,7 +120,7 @@ __global__ __launch_bounds__(kThreads) void fwd_stream(StreamArgs<T> a) {
       }
   };
 
-  int cur = a.span2[rb];
+  int cur = RS[rb - r0];
 #pragma unroll
   for (int q = 0; q <= P2; ++q) column(cur - P2 + q, V[q]);
 
@@ -110,7 +128,7 @@ __global__ __launch_bounds__(kThreads) void fwd_stream(StreamArgs<T> a) {
                           static_cast<long long>(a.m1) * a.m2 * j3 +
                           a.nsamp * a.c0;
   for (int j2 = rb; j2 < re; ++j2) {
-    const int s2 = a.span2[j2];
+    const int s2 = RS[j2 - r0];
     if (s2 != cur) {
       if (s2 - cur > P2) {
         cur = s2;
@@ -133,7 +151,7 @@ __global__ __launch_bounds__(kThreads) void fwd_stream(StreamArgs<T> a) {
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
       if (f >= a.nf) break;
-      const T* b2p = a.tab2[f] + static_cast<size_t>(j2) * (P2 + 1);
+      const T* b2p = RT + (static_cast<size_t>(f) * chunk_cap + (j2 - r0)) * (P2 + 1);
       T bb[P2 + 1];
 #pragma unroll
       for (int q = 0; q <= P2; ++q) bb[q] = b2p[q];
@@ -287,6 +305,20 @@ const double* ktab<double>(const DimDev& d, int order) {
   return d.ktab64 + static_cast<size_t>(order) * d.m * (d.p + 1);
 }
 
+}  // namespace
+
+// LDS of fwd_stream: G for nc (+1 weight) channels, then the chunk's B2 rows
+// of nf fields and its spans.
+size_t fwd_stream_lds(const Plan& pl, int nc, int nf, size_t elem) {
+  const Tiling& t = pl.fwd;
+  const size_t p2 = static_cast<size_t>(pl.dims[1].p);
+  return (static_cast<size_t>(nc + (pl.nurbs ? 1 : 0)) * t.w1max * t.w2max +
+          static_cast<size_t>(nf) * t.chunk_rows() * (p2 + 1)) * elem +
+         static_cast<size_t>(t.chunk_rows()) * sizeof(int);
+}
+
+namespace {
+
 template <typename T>
 hipError_t forward_stream(const Plan& pl, int nf, const int32_t* orders,
                           const T* P, const T* W, T* const* outs,
@@ -306,6 +338,8 @@ hipError_t forward_stream(const Plan& pl, int nf, const int32_t* orders,
   a.m2 = d2.m;
   a.p1 = d1.p;
   a.rows_per_wave = pl.fwd.rows_per_wave;
+  a.wx = pl.fwd.wx;
+  a.gmax = pl.fwd.w1max * pl.fwd.w2max;
   a.ntx = pl.fwd.n_tiles1;
   a.nty = pl.fwd.n_chunks2;
   a.xcd_map = pl.fwd.xcd_map;
@@ -357,8 +391,7 @@ hipError_t forward_stream(const Plan& pl, int nf, const int32_t* orders,
       if (!k) return hipErrorInvalidValue;
       a.c0 = c0;
       a.src.c0 = c0;
-      const size_t lds = static_cast<size_t>(nc + (nurbs ? 1 : 0)) * pl.fwd.w1max *
-                         pl.fwd.w2max * sizeof(T);
+      const size_t lds = fwd_stream_lds(pl, nc, kf, sizeof(T));
       hipLaunchKernelGGL(k, grid, dim3(kThreads), lds, s, a);
       hipError_t e = hipGetLastError();
       if (e != hipSuccess) return e;

@@ -43,24 +43,26 @@ struct DimDev {
 // dimension 2, one workgroup per (tile, chunk, sample of dimension 3).
 struct Tiling {
   int rows_per_wave = 32;
+  int wx = 1;                 // waves side by side along dimension 1 (1, 2, 4)
   int n_tiles1 = 0, n_chunks2 = 0;
   int w1max = 0, w2max = 0;   // largest band windows of a tile / chunk
   int w2wave_max = 0;         // largest window of one wave's rows
   int xcd_map = 0;            // workgroup numbering, see xcd_logical_block
   // adjoint only: samples of dimension 3 per workgroup and their k3 windows
   int j3_per_block = 1, n_z = 1, w3max = 1;
-  // tile / chunk / z-range ranges whose windows hold each k1 / k2 / k3
-  int32_t* tile_of_k1 = nullptr;   // [n1][2]
-  int32_t* chunk_of_k2 = nullptr;  // [n2][2]
-  int32_t* z_of_k3 = nullptr;      // [n3][2]
-  int chunk_rows() const { return rows_per_wave * kWaves; }
+  // adjoint gather: for dimension d and control point k, the (block,
+  // offset-in-window) pairs of the workgroups whose windows hold k
+  int32_t* gptr[3] = {nullptr, nullptr, nullptr};  // [n_d + 1]
+  int32_t* gent[3] = {nullptr, nullptr, nullptr};  // [2 * entries]
+  int tile_width() const { return kLanes * wx; }
+  int chunk_rows() const { return rows_per_wave * (kWaves / wx); }
 };
 
 struct Plan {
   int nin = 0, nout = 0, nurbs = 0, device = 0;
   std::vector<DimDev> dims;
   bool stream_ok = false;     // streaming (sum-factorised) kernels apply
-  Tiling fwd, adj;
+  Tiling fwd, adj, mma;
   int nt_stores = 0;          // non-temporal stores of eval
   void* work = nullptr;       // adjoint partial slabs
   size_t work_bytes = 0;
@@ -122,6 +124,7 @@ hipError_t launch_copy_f64(const double* in, size_t count, double* out,
                            hipStream_t s);
 
 // ---- sg_forward.hip ----
+size_t fwd_stream_lds(const Plan& plan, int nc, int nf, size_t elem);
 // Fields: derivative orders [nf][nin]; one output pointer per field.
 hipError_t launch_forward(const Plan& plan, int nf, const int32_t* orders,
                           const void* P, const void* W, void* const* outs,

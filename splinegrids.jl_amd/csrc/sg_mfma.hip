@@ -37,6 +37,7 @@ struct MmaArgs {
   int m1, m2, p1, p2;
   int rows_per_chunk;
   int ntx, nty, xcd_map;
+  int gmax;            // LDS floats reserved per channel of G
   float s1, s2;        // 2^-e of dimension 1 / 2 tables at the requested orders
   int e12;             // e1 + e2
   long long nsamp;
@@ -86,7 +87,8 @@ __global__ __launch_bounds__(kThreads) void mma_stream(MmaArgs a) {
 
   const int lane = threadIdx.x & (kLanes - 1);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kLanes);
-  const FwdTile tile = fwd_tile(a.xcd_map, a.ntx, a.nty, a.rows_per_chunk, a.m1, a.m2, NIN);
+  const FwdTile tile = fwd_tile(a.xcd_map, a.ntx, a.nty, kLanes, a.rows_per_chunk, a.m1, a.m2,
+                                NIN);
   const int r0 = tile.r0, r1 = tile.r1, j3 = tile.j3;
   const int p1 = a.p1, p2 = a.p2;
   const int lo1 = a.span1[tile.j1_0] - p1;
@@ -95,8 +97,15 @@ __global__ __launch_bounds__(kThreads) void mma_stream(MmaArgs a) {
   const int hi2 = a.span2[r1 - 1];
   const int W2 = hi2 - lo2 + 1;
   const int cs = W1 * W2;
+  // the chunk's B2 rows and spans in LDS (no vector-memory loads in the
+  // row loop, whose stores would otherwise be waited for)
+  float* RT = G + static_cast<size_t>(NCH) * a.gmax;
+  int* RS = reinterpret_cast<int*>(RT + static_cast<size_t>(a.rows_per_chunk) * (p2 + 1));
 
   stage1_band_window<float, NIN, NC, NURBS>(G, a.src, j3, lo1, W1, lo2, W2);
+  for (int i = threadIdx.x; i < r1 - r0; i += kThreads) RS[i] = a.span2[r0 + i];
+  for (int i = threadIdx.x; i < (r1 - r0) * (p2 + 1); i += kThreads)
+    RT[i] = a.tab2[static_cast<size_t>(r0) * (p2 + 1) + i] * a.s2;
   __syncthreads();
 
   // per-channel power-of-two scale of the window: max|G_c| * 2^-e < 1
@@ -158,16 +167,15 @@ __global__ __launch_bounds__(kThreads) void mma_stream(MmaArgs a) {
                           a.nsamp * a.src.c0;
   for (int jb = r0; jb < r1; jb += 16) {
     const int jl = min(jb + 15, r1 - 1);
-    const int klo = a.span2[jb] - p2, khi = a.span2[jl];
+    const int klo = RS[jb - r0] - p2, khi = RS[jl - r0];
     // this lane's row of the A operand
     const bool rowok = jb + col < r1;
     const int j2a = rowok ? jb + col : r1 - 1;
-    const int base2 = a.span2[j2a] - p2;
+    const int base2 = RS[j2a - r0] - p2;
     float b2[PM + 1];
 #pragma unroll
     for (int t = 0; t <= PM; ++t)
-      b2[t] = (rowok && t <= p2) ? a.tab2[static_cast<size_t>(j2a) * (p2 + 1) + min(t, p2)] * a.s2
-                                 : 0.f;
+      b2[t] = (rowok && t <= p2) ? RT[(j2a - r0) * (p2 + 1) + min(t, p2)] : 0.f;
     float4_t acc[NCH];
 #pragma unroll
     for (int ch = 0; ch < NCH; ++ch) acc[ch] = float4_t{0.f, 0.f, 0.f, 0.f};
@@ -242,7 +250,7 @@ MmaKernel pick_mma(int tier, int nin, int nc, bool nurbs) {
 
 }  // namespace
 
-bool mma_supported(const Plan& pl) { return pl.stream_ok; }
+bool mma_supported(const Plan& pl) { return pl.stream_ok && pl.mma.n_tiles1 > 0; }
 
 hipError_t launch_forward_mma(const Plan& pl, int tier, int nf, const int32_t* orders,
                               const float* P, const float* W, float* const* outs,
@@ -262,10 +270,12 @@ hipError_t launch_forward_mma(const Plan& pl, int tier, int nf, const int32_t* o
   a.m2 = d2.m;
   a.p1 = d1.p;
   a.p2 = d2.p;
-  a.rows_per_chunk = pl.fwd.chunk_rows();
-  a.ntx = pl.fwd.n_tiles1;
-  a.nty = pl.fwd.n_chunks2;
-  a.xcd_map = pl.fwd.xcd_map;
+  const Tiling& tl = pl.mma;
+  a.rows_per_chunk = tl.chunk_rows();
+  a.ntx = tl.n_tiles1;
+  a.nty = tl.n_chunks2;
+  a.xcd_map = tl.xcd_map;
+  a.gmax = tl.w1max * tl.w2max;
   a.src.ncp = 1;
   a.nsamp = 1;
   for (int d = 0; d < nin; ++d) {
@@ -278,7 +288,7 @@ hipError_t launch_forward_mma(const Plan& pl, int tier, int nf, const int32_t* o
     a.src.p3 = pl.dims[2].p;
     gz = static_cast<unsigned>(pl.dims[2].m);
   }
-  const dim3 grid(pl.fwd.n_tiles1 * pl.fwd.n_chunks2 * gz);
+  const dim3 grid(tl.n_tiles1 * tl.n_chunks2 * gz);
   for (int f = 0; f < nf; ++f) {
     const int o1 = orders[f * nin + 0], o2 = orders[f * nin + 1];
     a.tab1 = d1.ktab32 + static_cast<size_t>(o1) * d1.m * (d1.p + 1);
@@ -297,8 +307,9 @@ hipError_t launch_forward_mma(const Plan& pl, int tier, int nf, const int32_t* o
       MmaKernel k = pick_mma(tier, nin, nc, nurbs);
       if (!k) return hipErrorInvalidValue;
       a.src.c0 = c0;
-      const size_t lds = static_cast<size_t>(nc + (nurbs ? 1 : 0)) * pl.fwd.w1max *
-                         pl.fwd.w2max * sizeof(float);
+      const size_t lds = (static_cast<size_t>(nc + (nurbs ? 1 : 0)) * tl.w1max * tl.w2max +
+                          static_cast<size_t>(tl.chunk_rows()) * (d2.p + 1)) * sizeof(float) +
+                         static_cast<size_t>(tl.chunk_rows()) * sizeof(int);
       hipLaunchKernelGGL(k, grid, dim3(kThreads), lds, s, a);
       hipError_t e = hipGetLastError();
       if (e != hipSuccess) return e;

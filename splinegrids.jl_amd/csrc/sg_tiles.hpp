@@ -70,19 +70,19 @@ __device__ inline void stage1_band_window(T* G, const BandSrc<T>& b, int j3, int
   }
 }
 
-// Tile of a forward workgroup: 64 samples of dimension 1 from j1_0, rows
-// [r0, r1) of dimension 2, sample j3 of dimension 3.
+// Tile of a forward workgroup: tile_width samples of dimension 1 from j1_0,
+// rows [r0, r1) of dimension 2, sample j3 of dimension 3.
 struct FwdTile {
   int j1_0, j1_last, r0, r1, j3;
 };
 
-__device__ inline FwdTile fwd_tile(int xcd_map, int ntx, int nty, int rows_per_chunk, int m1,
-                                   int m2, int nin) {
+__device__ inline FwdTile fwd_tile(int xcd_map, int ntx, int nty, int tile_width,
+                                   int rows_per_chunk, int m1, int m2, int nin) {
   const int L = xcd_logical_block(xcd_map, ntx);
   const int bx = L % ntx, by = (L / ntx) % nty;
   FwdTile t;
-  t.j1_0 = bx * kLanes;
-  t.j1_last = min(t.j1_0 + kLanes - 1, m1 - 1);
+  t.j1_0 = bx * tile_width;
+  t.j1_last = min(t.j1_0 + tile_width - 1, m1 - 1);
   t.r0 = by * rows_per_chunk;
   t.r1 = min(t.r0 + rows_per_chunk, m2);
   t.j3 = nin == 3 ? L / (ntx * nty) : 0;
