@@ -13,6 +13,7 @@ for profiling and parity runs.
 from __future__ import annotations
 
 import argparse
+import gc
 import json
 import math
 import os
@@ -274,6 +275,16 @@ def batch_bytes(grids) -> int:
     return total
 
 
+def settle_gc() -> None:
+    """Collect once and freeze what set-up left (torch, the grids), so that no
+    full collection scans those objects in the middle of the timed loop; the
+    collector stays on for the loop's own objects.  A full collection after
+    building a large grid takes tens of milliseconds, more than a hundred
+    launches of a short kernel."""
+    gc.collect()
+    gc.freeze()
+
+
 def bench_batch(args, sg, wl):
     """C5: the per-grid launch loop, the batched persistent launch, and the
     batched launch captured in a hipGraph; value is the graph replay."""
@@ -291,6 +302,7 @@ def bench_batch(args, sg, wl):
         for _ in range(args.warmup):
             fn()
         torch.cuda.synchronize()
+        settle_gc()
         a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         a.record()
         for _ in range(steps):
@@ -382,6 +394,7 @@ def main(argv=None):
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
+    settle_gc()
     stream = torch.cuda.current_stream()
     start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     t_wall = time.perf_counter()

@@ -12,7 +12,7 @@
 //     A  each lane (sample j1) walks its wave's rows j2 and accumulates
 //        U[k2] += B2[j2, b] * ebar[j1, j2, j3] in a sliding window of p2+1
 //        registers; an entry that leaves the window is final and goes to
-//        the wave's LDS slab.  Loads are issued four rows ahead.
+//        the wave's LDS slab.  Row loads are double-buffered in batches of 4.
 //     B  the workgroup contracts the slabs along dimension 1: output
 //        (k1, k2) sums the lanes whose band holds k1 (B1 transposed per k1
 //        in LDS, 16 bytes per LDS read).
@@ -72,7 +72,7 @@ __device__ inline double dot_lanes(const double* a, const double* b, int l0, int
   return s;
 }
 
-constexpr int kRowBatch = 4;  // rows whose loads are in flight together
+constexpr int kRowBatch = 4;  // rows per load batch; two batches are in flight
 
 // LDS rows of 64 lanes are padded by 16 bytes so that rows read by
 // neighbouring threads start in different banks.
@@ -149,9 +149,13 @@ __global__ __launch_bounds__(kThreads) void adj_stream(AdjStreamArgs<T> a) {
     lhi[q1] = (lo_b + 3) & ~3;
   }
   for (int i = threadIdx.x; i < (p3 + 1) * slab; i += kThreads) R3[i] = T(0);
-  for (int i = threadIdx.x; i < r1 - r0; i += kThreads) RS[i] = a.span2[r0 + i];
-  for (int i = threadIdx.x; i < (r1 - r0) * (P2 + 1); i += kThreads)
-    RT[i] = a.tab2[static_cast<size_t>(r0) * (P2 + 1) + i];
+  {
+    const size_t rbase = static_cast<size_t>(r0) * (P2 + 1);
+    stage_to_lds<1, 1>(r1 - r0, [&](int, int i) { return a.span2[r0 + i]; },
+                       [&](int, int i, int v) { RS[i] = v; });
+    stage_to_lds<1, 4>((r1 - r0) * (P2 + 1), [&](int, int i) { return a.tab2[rbase + i]; },
+                       [&](int, int i, T v) { RT[i] = v; });
+  }
 
   // partial window of this workgroup: [w3max][nout][w2max][w1max]
   const size_t plane = static_cast<size_t>(a.w1max) * a.w2max;
@@ -182,7 +186,7 @@ __global__ __launch_bounds__(kThreads) void adj_stream(AdjStreamArgs<T> a) {
 
     // ---- A: stream the wave's rows, sliding window over k2 ----
     if (rb < re) {
-      int cur = RS[rb - r0];
+      int cur = wave_uniform(RS[rb - r0]);
       const int w0 = cur - P2;
       T U[P2 + 1][NC];
 #pragma unroll
@@ -190,8 +194,7 @@ __global__ __launch_bounds__(kThreads) void adj_stream(AdjStreamArgs<T> a) {
 #pragma unroll
         for (int c = 0; c < NC; ++c) U[q][c] = T(0);
       const long long ebase = static_cast<long long>(j1) + a.m12 * j3 + a.nsamp * a.c0;
-      for (int jb = rb; jb < re; jb += kRowBatch) {
-        T e[kRowBatch][NC];
+      auto load = [&](T (&e)[kRowBatch][NC], int jb) {
 #pragma unroll
         for (int r = 0; r < kRowBatch; ++r) {
           const bool ok = active && (jb + r < re);
@@ -199,11 +202,13 @@ __global__ __launch_bounds__(kThreads) void adj_stream(AdjStreamArgs<T> a) {
 #pragma unroll
           for (int c = 0; c < NC; ++c) e[r][c] = ok ? a.E[eoff + a.nsamp * c] : T(0);
         }
+      };
+      auto consume = [&](const T (&e)[kRowBatch][NC], int jb) {
 #pragma unroll
         for (int r = 0; r < kRowBatch; ++r) {
           const int j2 = jb + r;
           if (j2 >= re) break;
-          const int s2 = RS[j2 - r0];
+          const int s2 = wave_uniform(RS[j2 - r0]);
           while (cur < s2) {
             const int row = cur - P2 - w0;
 #pragma unroll
@@ -219,12 +224,20 @@ __global__ __launch_bounds__(kThreads) void adj_stream(AdjStreamArgs<T> a) {
           const T* b2p = RT + static_cast<size_t>(j2 - r0) * (P2 + 1);
           T bb[P2 + 1];
 #pragma unroll
-          for (int q = 0; q <= P2; ++q) bb[q] = b2p[q];
+          for (int q = 0; q <= P2; ++q) bb[q] = wave_uniform(b2p[q]);
 #pragma unroll
           for (int c = 0; c < NC; ++c)
 #pragma unroll
             for (int q = 0; q <= P2; ++q) U[q][c] += bb[q] * e[r][c];
         }
+      };
+      T ea[kRowBatch][NC], eb[kRowBatch][NC];
+      load(ea, rb);
+      for (int jb = rb; jb < re; jb += 2 * kRowBatch) {
+        if (jb + kRowBatch < re) load(eb, jb + kRowBatch);
+        consume(ea, jb);
+        if (jb + 2 * kRowBatch < re) load(ea, jb + 2 * kRowBatch);
+        if (jb + kRowBatch < re) consume(eb, jb + kRowBatch);
       }
 #pragma unroll
       for (int q = 0; q <= P2; ++q) {

@@ -48,8 +48,7 @@ __global__ void batch_reset(int* counter) {
 }
 
 template <int P2>
-__device__ void batch_tile(const BGrid& g, const BTile& t, float* smem, int gcap, int lane,
-                           int wave) {
+__device__ void batch_tile(const BGrid& g, const BTile& t, float* smem, int lane, int wave) {
   constexpr int PM = kMaxStreamDegree;
   const int chunk_cap = g.rows_per_wave * kWaves;
   const int j1_0 = t.bx * kLanes;
@@ -61,25 +60,35 @@ __device__ void batch_tile(const BGrid& g, const BTile& t, float* smem, int gcap
   const int lo2 = g.span2[r0] - P2;
   const int W2 = g.span2[r1 - 1] - lo2 + 1;
   const int cs = W1 * W2;
-  float* G = smem;                                        // [5][gcap]
-  float* RT = smem + static_cast<size_t>(kBC + 1) * gcap;  // [chunk][P2+1]
-  int* RS = reinterpret_cast<int*>(RT + static_cast<size_t>(chunk_cap) * (P2 + 1));
   const bool nurbs = g.nurbs != 0;
+  const int nch = min(kBC, g.nout) + (nurbs ? 1 : 0);
+  // [nch][cs] window slabs (weight last), then the chunk's B2 rows and spans
+  float* G = smem;
+  float* RT = smem + static_cast<size_t>(nch) * cs;  // [chunk][P2+1]
+  int* RS = reinterpret_cast<int*>(RT + static_cast<size_t>(chunk_cap) * (P2 + 1));
   const int j3 = t.j3;
 
-  for (int i = threadIdx.x; i < r1 - r0; i += kThreads) RS[i] = g.span2[r0 + i];
-  for (int i = threadIdx.x; i < (r1 - r0) * (P2 + 1); i += kThreads)
-    RT[i] = g.tab2[static_cast<size_t>(r0) * (P2 + 1) + i];
+  {
+    const size_t rbase = static_cast<size_t>(r0) * (P2 + 1);
+    stage_to_lds<1, 1>(r1 - r0, [&](int, int i) { return g.span2[r0 + i]; },
+                       [&](int, int i, int v) { RS[i] = v; });
+    stage_to_lds<1, 4>((r1 - r0) * (P2 + 1), [&](int, int i) { return g.tab2[rbase + i]; },
+                       [&](int, int i, float v) { RT[i] = v; });
+  }
 
   const int rb = r0 + wave * g.rows_per_wave;
   const int re = min(rb + g.rows_per_wave, r1);
   const bool active = (j1_0 + lane) < g.m1;
   const int j1 = active ? j1_0 + lane : g.m1 - 1;
   const int q1b = g.span1[j1] - p1 - lo1;
+  // b1 is zero past p1, so the window sums take all PM+1 terms without a
+  // branch (the clamped index re-reads the last element of the band)
   float b1[PM + 1];
 #pragma unroll
-  for (int q = 0; q <= PM; ++q)
-    b1[q] = (q <= p1) ? g.tab1[static_cast<size_t>(j1) * (p1 + 1) + min(q, p1)] : 0.f;
+  for (int q = 0; q <= PM; ++q) {
+    const float v = g.tab1[static_cast<size_t>(j1) * (p1 + 1) + min(q, p1)];
+    b1[q] = q <= p1 ? v : 0.f;
+  }
 
   for (int c0 = 0; c0 < g.nout; c0 += kBC) {
     const int nc = min(kBC, g.nout - c0);
@@ -113,8 +122,8 @@ __device__ void batch_tile(const BGrid& g, const BTile& t, float* smem, int gcap
       }
 #pragma unroll
       for (int c = 0; c < kBC; ++c)
-        if (c < nc) G[c * gcap + idx] = acc[c];
-      if (nurbs) G[kBC * gcap + idx] = accw;
+        if (c < nc) G[c * cs + idx] = acc[c];
+      if (nurbs) G[nc * cs + idx] = accw;
     }
     __syncthreads();
 
@@ -125,22 +134,23 @@ __device__ void batch_tile(const BGrid& g, const BTile& t, float* smem, int gcap
         const float* gr = G + (k2 - lo2) * W1 + q1b;
 #pragma unroll
         for (int ch = 0; ch <= kBC; ++ch) {
+          // register slot kBC holds the weight, whose LDS slab is nc
+          const int slab = ch == kBC ? nc : ch;
           float s = 0.f;
           if (ch < nc || (ch == kBC && nurbs)) {
 #pragma unroll
-            for (int q = 0; q <= PM; ++q)
-              if (q <= p1) s += b1[q] * gr[ch * gcap + min(q, p1)];
+            for (int q = 0; q <= PM; ++q) s += b1[q] * gr[slab * cs + min(q, p1)];
           }
           v[ch] = s;
         }
       };
-      int cur = RS[rb - r0];
+      int cur = wave_uniform(RS[rb - r0]);
 #pragma unroll
       for (int q = 0; q <= P2; ++q) column(cur - P2 + q, V[q]);
       const long long obase = static_cast<long long>(j1) +
                               static_cast<long long>(g.m1) * g.m2 * j3 + g.nsamp * c0;
       for (int j2 = rb; j2 < re; ++j2) {
-        const int s2 = RS[j2 - r0];
+        const int s2 = wave_uniform(RS[j2 - r0]);
         if (s2 != cur) {
           if (s2 - cur > P2) {
             cur = s2;
@@ -157,7 +167,10 @@ __device__ void batch_tile(const BGrid& g, const BTile& t, float* smem, int gcap
             }
           }
         }
-        const float* bb = RT + static_cast<size_t>(j2 - r0) * (P2 + 1);
+        const float* b2p = RT + static_cast<size_t>(j2 - r0) * (P2 + 1);
+        float bb[P2 + 1];
+#pragma unroll
+        for (int q = 0; q <= P2; ++q) bb[q] = wave_uniform(b2p[q]);
         float num[kBC + 1];
 #pragma unroll
         for (int ch = 0; ch <= kBC; ++ch) {
@@ -180,8 +193,7 @@ __device__ void batch_tile(const BGrid& g, const BTile& t, float* smem, int gcap
 
 __global__ __launch_bounds__(kThreads) void batch_persistent(const BGrid* __restrict__ grids,
                                                              const BTile* __restrict__ tiles,
-                                                             int n_tiles, int gcap,
-                                                             int* counter) {
+                                                             int n_tiles, int* counter) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   float* smem = reinterpret_cast<float*>(smem_raw);
   __shared__ int s_tile;
@@ -195,12 +207,12 @@ __global__ __launch_bounds__(kThreads) void batch_persistent(const BGrid* __rest
     const BTile t = tiles[ti];
     const BGrid g = grids[t.grid];
     switch (g.p2) {
-      case 0: batch_tile<0>(g, t, smem, gcap, lane, wave); break;
-      case 1: batch_tile<1>(g, t, smem, gcap, lane, wave); break;
-      case 2: batch_tile<2>(g, t, smem, gcap, lane, wave); break;
-      case 3: batch_tile<3>(g, t, smem, gcap, lane, wave); break;
-      case 4: batch_tile<4>(g, t, smem, gcap, lane, wave); break;
-      default: batch_tile<5>(g, t, smem, gcap, lane, wave); break;
+      case 0: batch_tile<0>(g, t, smem, lane, wave); break;
+      case 1: batch_tile<1>(g, t, smem, lane, wave); break;
+      case 2: batch_tile<2>(g, t, smem, lane, wave); break;
+      case 3: batch_tile<3>(g, t, smem, lane, wave); break;
+      case 4: batch_tile<4>(g, t, smem, lane, wave); break;
+      default: batch_tile<5>(g, t, smem, lane, wave); break;
     }
     // batch_tile ends with a barrier, so s_tile may be overwritten now
   }
@@ -214,14 +226,15 @@ using namespace sg;
 
 struct sg_batch {
   std::vector<Plan*> plans;       // one per grid
-  std::vector<int> persistent;    // grids the persistent kernel evaluates
+  std::vector<int> candidates;    // grids the persistent kernel can take
+  std::vector<int> persistent;    // ... and takes under the current tiling
+  std::vector<int> fixed_separate;  // grids that always need their own launch
   std::vector<int> separate;      // grids evaluated with their own launches
   std::vector<BGrid> hgrids;
   BGrid* dgrids = nullptr;
   BTile* dtiles = nullptr;
   int* counter = nullptr;
   int n_tiles = 0;
-  int gcap = 0;                   // LDS floats per G slab
   size_t lds = 0;
   int rows_per_wave = 16;
   int blocks_per_cu = 4;
@@ -243,42 +256,64 @@ void free_batch(sg_batch* b) {
   delete b;
 }
 
-// Tiles of every persistent grid for a rows-per-wave choice; largest first.
+// Window sizes of a grid's tiles for a rows-per-wave choice.
+struct GridTiling {
+  int rows, ntx, nty, w1, w2;
+  size_t lds;  // bytes of LDS one tile of this grid needs
+};
+
+GridTiling grid_tiling(const Plan& pl, int rows) {
+  const DimDev& d1 = pl.dims[0];
+  const DimDev& d2 = pl.dims[1];
+  GridTiling t{rows, (d1.m + kLanes - 1) / kLanes, 0, 0, 0, 0};
+  const int chunk = rows * kWaves;
+  t.nty = (d2.m + chunk - 1) / chunk;
+  for (int i = 0; i < t.ntx; ++i) {
+    const int a = i * kLanes, e = std::min(a + kLanes, d1.m) - 1;
+    t.w1 = std::max(t.w1, d1.h_span[e] - d1.h_span[a] + d1.p + 1);
+  }
+  for (int c = 0; c < t.nty; ++c) {
+    const int a = c * chunk, e = std::min(a + chunk, d2.m) - 1;
+    t.w2 = std::max(t.w2, d2.h_span[e] - d2.h_span[a] + d2.p + 1);
+  }
+  const size_t nch = std::min(pl.nout, kBC) + (pl.nurbs ? 1 : 0);
+  t.lds = (nch * t.w1 * t.w2 + static_cast<size_t>(chunk) * (d2.p + 2)) * sizeof(float);
+  return t;
+}
+
+// Tiles of every persistent grid, largest first.  Each grid takes the
+// batch's rows per wave, halved until its tiles fit in LDS; a grid whose
+// tiles never fit is evaluated with its own launch instead.
 int build_tiles(sg_batch* b, int rows_per_wave) {
   std::vector<BTile> tiles;
   std::vector<long long> cost;
-  int gcap = 1;
-  size_t row_words = 0;
-  for (int gi : b->persistent) {
+  size_t lds = 16;
+  std::vector<int> persistent;
+  b->separate = b->fixed_separate;
+  for (int gi : b->candidates) {
     const Plan& pl = *b->plans[gi];
-    Tiling t;
+    int rows = rows_per_wave;
+    GridTiling t = grid_tiling(pl, rows);
+    while (t.lds > static_cast<size_t>(kMaxLds) && rows > 1) t = grid_tiling(pl, rows /= 2);
+    if (t.lds > static_cast<size_t>(kMaxLds)) {
+      b->separate.push_back(gi);
+      continue;
+    }
+    persistent.push_back(gi);
+    lds = std::max(lds, t.lds);
     const DimDev& d1 = pl.dims[0];
     const DimDev& d2 = pl.dims[1];
-    t.rows_per_wave = rows_per_wave;
-    t.n_tiles1 = (d1.m + kLanes - 1) / kLanes;
-    const int chunk = rows_per_wave * kWaves;
-    t.n_chunks2 = (d2.m + chunk - 1) / chunk;
-    int w1 = 0, w2 = 0;
-    for (int i = 0; i < t.n_tiles1; ++i) {
-      const int a = i * kLanes, e = std::min(a + kLanes, d1.m) - 1;
-      w1 = std::max(w1, d1.h_span[e] - d1.h_span[a] + d1.p + 1);
-    }
-    for (int c = 0; c < t.n_chunks2; ++c) {
-      const int a = c * chunk, e = std::min(a + chunk, d2.m) - 1;
-      w2 = std::max(w2, d2.h_span[e] - d2.h_span[a] + d2.p + 1);
-    }
-    gcap = std::max(gcap, w1 * w2);
-    row_words = std::max(row_words, static_cast<size_t>(chunk) * (d2.p + 2));
+    const int chunk = rows * kWaves;
     const int m3 = pl.nin == 3 ? pl.dims[2].m : 1;
     for (int z = 0; z < m3; ++z)
-      for (int y = 0; y < t.n_chunks2; ++y)
-        for (int x = 0; x < t.n_tiles1; ++x) {
+      for (int y = 0; y < t.nty; ++y)
+        for (int x = 0; x < t.ntx; ++x) {
           tiles.push_back(BTile{gi, x, y, z});
           const long long lanes = std::min(kLanes, d1.m - x * kLanes);
-          const long long rows = std::min(chunk, d2.m - y * chunk);
-          cost.push_back(lanes * rows * pl.nout + 64 * w1 * w2);
+          const long long nrows = std::min(chunk, d2.m - y * chunk);
+          cost.push_back(lanes * nrows * pl.nout + 64LL * t.w1 * t.w2);
         }
-    b->hgrids[gi].rows_per_wave = rows_per_wave;
+    b->hgrids[gi].rows_per_wave = rows;
   }
   std::vector<size_t> order(tiles.size());
   std::iota(order.begin(), order.end(), 0);
@@ -286,8 +321,6 @@ int build_tiles(sg_batch* b, int rows_per_wave) {
                    [&](size_t a, size_t c) { return cost[a] > cost[c]; });
   std::vector<BTile> sorted(tiles.size());
   for (size_t i = 0; i < order.size(); ++i) sorted[i] = tiles[order[i]];
-  const size_t lds = (static_cast<size_t>(kBC + 1) * gcap + row_words) * sizeof(float);
-  if (lds > static_cast<size_t>(kMaxLds)) return fail(SG_ERR_UNSUPPORTED, "batch tiles exceed LDS");
   if (b->dtiles) (void)hipFree(b->dtiles);
   b->dtiles = nullptr;
   hipError_t e = hipSuccess;
@@ -300,8 +333,8 @@ int build_tiles(sg_batch* b, int rows_per_wave) {
     e = hipMemcpy(b->dgrids, b->hgrids.data(), b->hgrids.size() * sizeof(BGrid),
                   hipMemcpyHostToDevice);
   if (e != hipSuccess) return hip_fail(e, "batch tile upload");
+  b->persistent = persistent;
   b->n_tiles = static_cast<int>(sorted.size());
-  b->gcap = gcap;
   b->lds = lds;
   b->rows_per_wave = rows_per_wave;
   return SG_OK;
@@ -313,7 +346,7 @@ hipError_t launch_batch(sg_batch* b, hipStream_t s) {
   if (e != hipSuccess || b->n_tiles == 0) return e;
   const int blocks = std::max(1, std::min(b->n_tiles, b->blocks_per_cu * b->n_cu));
   hipLaunchKernelGGL(batch_persistent, dim3(blocks), dim3(kThreads), b->lds, s, b->dgrids,
-                     b->dtiles, b->n_tiles, b->gcap, b->counter);
+                     b->dtiles, b->n_tiles, b->counter);
   return hipGetLastError();
 }
 
@@ -356,7 +389,7 @@ int sg_batch_create(const sg_grid_desc* grids, int32_t n_grids, void* stream, sg
     for (const DimDev& d : pl.dims) ns *= d.m;
     b->n_samples += ns;
     const bool ok = pl.stream_ok && pl.dims[1].p <= kMaxStreamDegree;
-    (ok ? b->persistent : b->separate).push_back(i);
+    (ok ? b->candidates : b->fixed_separate).push_back(i);
     BGrid& g = b->hgrids[i];
     g.nin = pl.nin;
     g.nout = pl.nout;

@@ -78,11 +78,13 @@ __global__ __launch_bounds__(kThreads) void fwd_stream(StreamArgs<T> a) {
 
   // ---- stage 1 ----
   stage1_band_window<T, NIN, NC, NURBS>(G, a.src, j3, lo1, W1, lo2, W2);
-  for (int i = threadIdx.x; i < chunk; i += kThreads) RS[i] = a.span2[r0 + i];
-  for (int i = threadIdx.x; i < NF * chunk * (P2 + 1); i += kThreads) {
-    const int f = i / (chunk * (P2 + 1)), rest = i - f * chunk * (P2 + 1);
-    RT[static_cast<size_t>(f) * chunk_cap * (P2 + 1) + rest] =
-        a.tab2[f][static_cast<size_t>(r0) * (P2 + 1) + rest];
+  {
+    const size_t rbase = static_cast<size_t>(r0) * (P2 + 1);
+    const size_t fstride = static_cast<size_t>(chunk_cap) * (P2 + 1);
+    stage_to_lds<1, 1>(chunk, [&](int, int i) { return a.span2[r0 + i]; },
+                       [&](int, int i, int v) { RS[i] = v; });
+    stage_to_lds<NF, 4>(chunk * (P2 + 1), [&](int f, int i) { return a.tab2[f][rbase + i]; },
+                        [&](int f, int i, T v) { RT[f * fstride + i] = v; });
   }
   __syncthreads();
 
@@ -97,12 +99,16 @@ __global__ __launch_bounds__(kThreads) void fwd_stream(StreamArgs<T> a) {
   const int p1 = a.p1;
   const int q1b = a.span1[j1] - p1 - lo1;
 
+  // b1 is zero past p1, so the window sums below take all PM+1 terms without
+  // a branch (the clamped index re-reads the last element of the band)
   T b1[NF][PM + 1];
 #pragma unroll
   for (int f = 0; f < NF; ++f)
 #pragma unroll
-    for (int t = 0; t <= PM; ++t)
-      b1[f][t] = (t <= p1) ? a.tab1[f][static_cast<size_t>(j1) * (p1 + 1) + min(t, p1)] : T(0);
+    for (int t = 0; t <= PM; ++t) {
+      const T v = a.tab1[f][static_cast<size_t>(j1) * (p1 + 1) + min(t, p1)];
+      b1[f][t] = t <= p1 ? v : T(0);
+    }
 
   T V[P2 + 1][NF][NCH];
   // V[q] holds k2 = cur - P2 + q
@@ -114,13 +120,12 @@ __global__ __launch_bounds__(kThreads) void fwd_stream(StreamArgs<T> a) {
       for (int ch = 0; ch < NCH; ++ch) {
         T s = T(0);
 #pragma unroll
-        for (int t = 0; t <= PM; ++t)
-          if (t <= p1) s += b1[f][t] * g[ch * cs + min(t, p1)];
+        for (int t = 0; t <= PM; ++t) s += b1[f][t] * g[ch * cs + min(t, p1)];
         v[f][ch] = s;
       }
   };
 
-  int cur = RS[rb - r0];
+  int cur = wave_uniform(RS[rb - r0]);
 #pragma unroll
   for (int q = 0; q <= P2; ++q) column(cur - P2 + q, V[q]);
 
@@ -128,7 +133,7 @@ __global__ __launch_bounds__(kThreads) void fwd_stream(StreamArgs<T> a) {
                           static_cast<long long>(a.m1) * a.m2 * j3 +
                           a.nsamp * a.c0;
   for (int j2 = rb; j2 < re; ++j2) {
-    const int s2 = RS[j2 - r0];
+    const int s2 = wave_uniform(RS[j2 - r0]);
     if (s2 != cur) {
       if (s2 - cur > P2) {
         cur = s2;
@@ -148,19 +153,23 @@ __global__ __launch_bounds__(kThreads) void fwd_stream(StreamArgs<T> a) {
       }
     }
     const long long orow = obase + static_cast<long long>(a.m1) * j2;
+    // B2 rows of every field first (slots of fields >= nf hold rows of
+    // field 0, see the launcher), so one LDS wait covers the row
+    T bb[NF][P2 + 1];
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+#pragma unroll
+      for (int q = 0; q <= P2; ++q)
+        bb[f][q] = wave_uniform(RT[(static_cast<size_t>(f) * chunk_cap + (j2 - r0)) * (P2 + 1) + q]);
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
       if (f >= a.nf) break;
-      const T* b2p = RT + (static_cast<size_t>(f) * chunk_cap + (j2 - r0)) * (P2 + 1);
-      T bb[P2 + 1];
-#pragma unroll
-      for (int q = 0; q <= P2; ++q) bb[q] = b2p[q];
       T num[NCH];
 #pragma unroll
       for (int ch = 0; ch < NCH; ++ch) {
         T s = T(0);
 #pragma unroll
-        for (int q = 0; q <= P2; ++q) s += bb[q] * V[q][f][ch];
+        for (int q = 0; q <= P2; ++q) s += bb[f][q] * V[q][f][ch];
         num[ch] = s;
       }
       if (active) {

@@ -21,6 +21,41 @@ constexpr int kThreads = kLanes * kWaves;
 constexpr int kMaxStreamDegree = 5;     // streaming kernels: p1, p2 <= 5
 constexpr int kMaxLds = 64 * 1024;      // LDS bytes one workgroup may claim
 
+// Workgroup copy of n >= 1 elements of NF sources into LDS:
+// store(f, i, load(f, i)) for f < NF, i < n.  Each thread issues all NF * U
+// loads of a pass before its first LDS write, so staging a tile costs one
+// memory latency per pass instead of one per element (a plain loop waits for
+// each load before its write).  Loads past n re-read element n - 1.
+template <int NF, int U, typename Load, typename Store>
+__device__ inline void stage_to_lds(int n, Load load, Store store) {
+  for (int i0 = threadIdx.x; i0 < n; i0 += U * kThreads) {
+    decltype(load(0, 0)) v[NF][U];
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+#pragma unroll
+      for (int u = 0; u < U; ++u) v[f][u] = load(f, min(i0 + u * kThreads, n - 1));
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (i0 + u * kThreads < n) store(f, i0 + u * kThreads, v[f][u]);
+  }
+}
+
+// A value that every lane of the wave holds (read from one LDS address),
+// moved to a scalar register: branches on it stay scalar branches and FMAs
+// take it as a scalar operand.
+__device__ inline int wave_uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ inline float wave_uniform(float v) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+__device__ inline double wave_uniform(double v) {
+  const long long b = __double_as_longlong(v);
+  const unsigned lo = static_cast<unsigned>(__builtin_amdgcn_readfirstlane(static_cast<int>(b)));
+  const int hi = __builtin_amdgcn_readfirstlane(static_cast<int>(b >> 32));
+  return __longlong_as_double((static_cast<long long>(hi) << 32) | lo);
+}
+
 struct DimDev {
   int p = 0, n = 0, m = 0, D = 0;
   double* knots = nullptr;    // [n + p + 1]

@@ -103,9 +103,13 @@ __global__ __launch_bounds__(kThreads) void mma_stream(MmaArgs a) {
   int* RS = reinterpret_cast<int*>(RT + static_cast<size_t>(a.rows_per_chunk) * (p2 + 1));
 
   stage1_band_window<float, NIN, NC, NURBS>(G, a.src, j3, lo1, W1, lo2, W2);
-  for (int i = threadIdx.x; i < r1 - r0; i += kThreads) RS[i] = a.span2[r0 + i];
-  for (int i = threadIdx.x; i < (r1 - r0) * (p2 + 1); i += kThreads)
-    RT[i] = a.tab2[static_cast<size_t>(r0) * (p2 + 1) + i] * a.s2;
+  {
+    const size_t rbase = static_cast<size_t>(r0) * (p2 + 1);
+    stage_to_lds<1, 1>(r1 - r0, [&](int, int i) { return a.span2[r0 + i]; },
+                       [&](int, int i, int v) { RS[i] = v; });
+    stage_to_lds<1, 4>((r1 - r0) * (p2 + 1), [&](int, int i) { return a.tab2[rbase + i]; },
+                       [&](int, int i, float v) { RT[i] = v * a.s2; });
+  }
   __syncthreads();
 
   // per-channel power-of-two scale of the window: max|G_c| * 2^-e < 1
