@@ -111,6 +111,11 @@ int sg_plan_basis(const sg_plan* plan, int32_t dim, double* table,
 /* Bytes of device memory the plan holds (tables + workspace). */
 int sg_plan_device_bytes(const sg_plan* plan, int64_t* bytes);
 
+/* One line of text naming the kernels and tile shapes the plan chose (for
+ * logs and tests; no counterpart in the reference), truncated to capacity-1
+ * characters and NUL-terminated. */
+int sg_plan_describe(const sg_plan* plan, char* text, int32_t capacity);
+
 /* evaluate!(grid; derivative_order, control_points, eval): replaces the
  * spline_eval_kernel launch of src/spline_grid.jl:200-230 (kernel :119-183).
  * eval is overwritten.  weights must be non-NULL iff the plan is NURBS, and

@@ -20,7 +20,7 @@ MAX_FIELDS = 4
 # every symbol include/splinegrids_hip.h declares
 EXPORTS = (
     "sg_last_error", "sg_version", "sg_dimension_evaluate", "sg_plan_create",
-    "sg_plan_destroy", "sg_plan_basis", "sg_plan_device_bytes", "sg_evaluate",
+    "sg_plan_destroy", "sg_plan_basis", "sg_plan_device_bytes", "sg_plan_describe", "sg_evaluate",
     "sg_evaluate_fields", "sg_evaluate_adjoint", "sg_stream_synchronize",
     "sg_batch_create", "sg_batch_destroy", "sg_batch_set_arrays", "sg_batch_evaluate",
     "sg_batch_evaluate_each", "sg_batch_autotune", "sg_batch_info",
@@ -77,6 +77,7 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
         "sg_plan_destroy": (ctypes.c_int, [vp]),
         "sg_plan_basis": (ctypes.c_int, [vp, i32, f64p, i32p]),
         "sg_plan_device_bytes": (ctypes.c_int, [vp, ctypes.POINTER(i64)]),
+        "sg_plan_describe": (ctypes.c_int, [vp, ctypes.c_char_p, i32]),
         "sg_evaluate": (ctypes.c_int, [vp, i32p, vp, vp, vp, i32, i32, vp]),
         "sg_evaluate_fields": (ctypes.c_int, [vp, i32, i32p, vp, ctypes.POINTER(vp), i32, i32, vp]),
         "sg_evaluate_adjoint": (ctypes.c_int, [vp, i32p, vp, vp, i32, i32, i32, vp]),

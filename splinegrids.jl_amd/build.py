@@ -16,7 +16,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libsplinegrids_hip.so")
-SOURCES = ["sg_basis.hip", "sg_forward.hip", "sg_mfma.hip", "sg_adjoint.hip", "sg_batch.hip", "sg_api.cpp"]
+SOURCES = ["sg_basis.hip", "sg_forward.hip", "sg_mfma.hip", "sg_adjoint.hip", "sg_adjoint_mma.hip",
+           "sg_batch.hip", "sg_api.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

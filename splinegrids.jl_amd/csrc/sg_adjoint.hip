@@ -303,34 +303,54 @@ struct AdjGatherArgs {
   long long ncp;
 };
 
+// Lanes per control point in adj_gather: lane `sub` of a group sums the
+// windows sub, sub + kGatherLanes, ... (kBatch loads in flight), then the
+// group adds its partial sums with a fixed butterfly, so every result is
+// the same sum in the same order on every run.
+constexpr int kGatherLanes = 4;
+
 template <typename T>
 __global__ __launch_bounds__(256) void adj_gather(AdjGatherArgs<T> a) {
-  const long long idx = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
-  if (idx >= a.ncp * a.nout) return;
-  const int c = static_cast<int>(idx / a.ncp);
-  long long r = idx - static_cast<long long>(c) * a.ncp;
+  const long long gidx = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
+  const long long idx = gidx / kGatherLanes;
+  const int sub = static_cast<int>(gidx % kGatherLanes);
+  const bool live = idx < a.ncp * a.nout;
+  const long long cidx = live ? idx : 0;  // dead lanes join the butterfly
+  const int c = static_cast<int>(cidx / a.ncp);
+  long long r = cidx - static_cast<long long>(c) * a.ncp;
   const int k1 = static_cast<int>(r % a.n1);
   r /= a.n1;
   const int k2 = static_cast<int>(r % a.n2);
   const int k3 = static_cast<int>(r / a.n2);
   const size_t plane = static_cast<size_t>(a.w1max) * a.w2max;
-  const int e1a = a.ptr1[k1], e1b = a.ptr1[k1 + 1];
-  const int e2a = a.ptr2[k2], e2b = a.ptr2[k2 + 1];
-  const int e3a = a.ptr3[k3], e3b = a.ptr3[k3 + 1];
+  const int e1a = a.ptr1[k1], n1e = a.ptr1[k1 + 1] - e1a;
+  const int e2a = a.ptr2[k2], n2e = a.ptr2[k2 + 1] - e2a;
+  const int e3a = a.ptr3[k3], n3e = a.ptr3[k3 + 1] - e3a;
+  // the n1e * n2e * n3e windows that hold (k1, k2, k3), dimension 1 fastest
+  constexpr int kBatch = 4;
+  const int total = live ? n1e * n2e * n3e : 0;
   T sum = T(0);
-  for (int e3 = e3a; e3 < e3b; ++e3) {
-    const int2 z = a.ent3[e3];
-    for (int e2 = e2a; e2 < e2b; ++e2) {
-      const int2 y = a.ent2[e2];
-      const size_t row = ((static_cast<size_t>(z.x) * a.nty + y.x) * a.ntx) * a.w3max;
-      for (int e1 = e1a; e1 < e1b; ++e1) {
-        const int2 x = a.ent1[e1];
-        const size_t unit_row = row + static_cast<size_t>(x.x) * a.w3max + z.y;
-        sum += a.part[(unit_row * a.nout + c) * plane + static_cast<size_t>(y.y) * a.w1max + x.y];
-      }
+  for (int f0 = sub; f0 < total; f0 += kBatch * kGatherLanes) {
+    T v[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      const int f = min(f0 + u * kGatherLanes, total - 1);
+      const int i1 = f % n1e, rest = f / n1e;
+      const int i2 = rest % n2e, i3 = rest / n2e;
+      const int2 x = a.ent1[e1a + i1];
+      const int2 y = a.ent2[e2a + i2];
+      const int2 z = a.ent3[e3a + i3];
+      const size_t unit =
+          ((static_cast<size_t>(z.x) * a.nty + y.x) * a.ntx + x.x) * a.w3max + z.y;
+      v[u] = a.part[(unit * a.nout + c) * plane + static_cast<size_t>(y.y) * a.w1max + x.y];
     }
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u)
+      if (f0 + u * kGatherLanes < total) sum += v[u];
   }
-  a.out[idx] = a.accumulate ? a.out[idx] + sum : sum;
+#pragma unroll
+  for (int m = kGatherLanes / 2; m >= 1; m /= 2) sum += __shfl_xor(sum, m, kGatherLanes);
+  if (live && sub == 0) a.out[idx] = a.accumulate ? a.out[idx] + sum : sum;
 }
 
 template <typename T>
@@ -510,8 +530,15 @@ hipError_t adjoint_stream(const Plan& pl, const int32_t* orders, const T* E,
     c0 += nc;
   }
 
+  return launch_adjoint_gather(pl, tl, tl.w1max, tl.w2max, pl.work, Pbar,
+                               sizeof(T) == 4 ? SG_F32 : SG_F64, accumulate, s);
+}
+
+template <typename T>
+hipError_t gather(const Plan& pl, const Tiling& tl, int w1, int w2, const T* part, T* Pbar,
+                  int accumulate, hipStream_t s) {
   AdjGatherArgs<T> g{};
-  g.part = static_cast<const T*>(pl.work);
+  g.part = part;
   g.out = Pbar;
   g.ptr1 = tl.gptr[0];
   g.ent1 = reinterpret_cast<const int2*>(tl.gent[0]);
@@ -519,18 +546,18 @@ hipError_t adjoint_stream(const Plan& pl, const int32_t* orders, const T* E,
   g.ent2 = reinterpret_cast<const int2*>(tl.gent[1]);
   g.ptr3 = tl.gptr[2];
   g.ent3 = reinterpret_cast<const int2*>(tl.gent[2]);
-  g.n1 = d1.n;
-  g.n2 = d2.n;
+  g.n1 = pl.dims[0].n;
+  g.n2 = pl.dims[1].n;
   g.nout = pl.nout;
-  g.w1max = tl.w1max;
-  g.w2max = tl.w2max;
+  g.w1max = w1;
+  g.w2max = w2;
   g.w3max = tl.w3max;
   g.ntx = tl.n_tiles1;
   g.nty = tl.n_chunks2;
   g.accumulate = accumulate;
   g.ncp = 1;
-  for (int d = 0; d < nin; ++d) g.ncp *= pl.dims[d].n;
-  const long long total = g.ncp * pl.nout;
+  for (int d = 0; d < pl.nin; ++d) g.ncp *= pl.dims[d].n;
+  const long long total = g.ncp * pl.nout * kGatherLanes;
   hipLaunchKernelGGL(adj_gather<T>, dim3(static_cast<unsigned>((total + 255) / 256)),
                      dim3(256), 0, s, g);
   return hipGetLastError();
@@ -566,6 +593,16 @@ hipError_t adjoint_csr(const Plan& pl, const int32_t* orders, const T* E,
 
 }  // namespace
 
+hipError_t launch_adjoint_gather(const Plan& pl, const Tiling& tl, int w1, int w2,
+                                 const void* part, void* Pbar, int dtype, int accumulate,
+                                 hipStream_t s) {
+  if (dtype == SG_F32)
+    return gather<float>(pl, tl, w1, w2, static_cast<const float*>(part),
+                         static_cast<float*>(Pbar), accumulate, s);
+  return gather<double>(pl, tl, w1, w2, static_cast<const double*>(part),
+                        static_cast<double*>(Pbar), accumulate, s);
+}
+
 size_t adjoint_work_bytes(const Plan& pl) {
   const Tiling& t = pl.adj;
   if (t.n_tiles1 == 0) return 0;
@@ -587,6 +624,8 @@ hipError_t launch_adjoint(const Plan& pl, const int32_t* orders,
   if (dtype == SG_F32) {
     auto e = static_cast<const float*>(ebar);
     auto o = static_cast<float*>(Pbar);
+    if (pl.adjm.n_tiles1 > 0 && nsamp > 0)
+      return launch_adjoint_mma(pl, orders, e, o, accumulate, s);
     return stream ? adjoint_stream<float>(pl, orders, e, o, accumulate, s)
                   : adjoint_csr<float>(pl, orders, e, o, accumulate, s);
   }

@@ -254,6 +254,83 @@ int validate_desc(const sg_grid_desc* desc) {
   return SG_OK;
 }
 
+// For every dimension d and control point k, the (block, offset) pairs of the
+// blocks of tiling tl whose band windows hold k: the lists adj_gather walks.
+static hipError_t upload_gather_lists(Plan& pl, Tiling& tl) {
+  auto lists = [&](int nblocks, int block_len, const DimDev& D, std::vector<int32_t>& ptr,
+                   std::vector<int32_t>& ent) {
+    std::vector<std::vector<int32_t>> per(D.n);
+    for (int b = 0; b < nblocks; ++b) {
+      const int a = b * block_len, e = std::min(a + block_len, D.m) - 1;
+      const int lo = D.h_span[a] - D.p;
+      for (int k = lo; k <= D.h_span[e]; ++k) {
+        per[k].push_back(b);
+        per[k].push_back(k - lo);
+      }
+    }
+    ptr.assign(D.n + 1, 0);
+    ent.clear();
+    for (int k = 0; k < D.n; ++k) {
+      ent.insert(ent.end(), per[k].begin(), per[k].end());
+      ptr[k + 1] = static_cast<int32_t>(ent.size() / 2);
+    }
+  };
+  std::vector<int32_t> ptr[3], ent[3];
+  lists(tl.n_tiles1, tl.tile_width(), pl.dims[0], ptr[0], ent[0]);
+  lists(tl.n_chunks2, tl.chunk_rows(), pl.dims[1], ptr[1], ent[1]);
+  if (pl.nin == 3) {
+    lists(tl.n_z, tl.j3_per_block, pl.dims[2], ptr[2], ent[2]);
+  } else {
+    ptr[2] = {0, 1};
+    ent[2] = {0, 0};
+  }
+  hipError_t e = hipSuccess;
+  for (int d = 0; d < 3 && e == hipSuccess; ++d) {
+    e = dalloc(pl, &tl.gptr[d], ptr[d].size());
+    if (e == hipSuccess) e = dalloc(pl, &tl.gent[d], ent[d].size());
+    if (e == hipSuccess)
+      e = hipMemcpy(tl.gptr[d], ptr[d].data(), ptr[d].size() * sizeof(int32_t),
+                    hipMemcpyHostToDevice);
+    if (e == hipSuccess && !ent[d].empty())
+      e = hipMemcpy(tl.gent[d], ent[d].data(), ent[d].size() * sizeof(int32_t),
+                    hipMemcpyHostToDevice);
+  }
+  return e;
+}
+
+// Matrix-core adjoint (FP32): 64-sample tiles along dimension 1 and chunks
+// of 64, 32 or 16 rows along dimension 2 whose band windows fit one 16 x 16 MFMA
+// tile; samples of dimension 3 per task chosen so that all tasks (one wave
+// each) are resident at once.  Leaves pl.adjm empty when the bands do not fit.
+static void plan_adjoint_mma(Plan& pl) {
+  Tiling& t = pl.adjm;
+  t = Tiling();
+  t.tile_w = kLanes;
+  bool fits = false;
+  for (int ch : {64, 32, 16}) {
+    t.chunk_h = ch;
+    compute_tiling(pl, t, ch / kWaves);
+    if (t.w1max > 16) break;
+    if (t.w2max <= 16) {
+      fits = true;
+      break;
+    }
+  }
+  if (!fits) {
+    t = Tiling();
+    return;
+  }
+  int j3 = env_int("SG_ADJM_J3", 0);
+  if (j3 <= 0 && pl.nin == 3) {
+    const long long xy = static_cast<long long>(t.n_tiles1) * t.n_chunks2;
+    const long long m3 = pl.dims[2].m;
+    const long long slots = std::max(adjoint_mma_resident_waves(pl), 1);
+    j3 = static_cast<int>(std::max<long long>(1, (xy * m3 + slots - 1) / slots));
+    while (j3 < m3 && xy * ((m3 + j3 - 1) / j3) > slots) ++j3;
+  }
+  compute_ztiling(pl, t, j3);
+}
+
 // Build a plan for a validated descriptor on the current device.  On success
 // *result owns every allocation; on failure everything is freed.
 int build_plan(const sg_grid_desc* desc, hipStream_t s, bool want_adjoint, Plan** result) {
@@ -391,53 +468,24 @@ int build_plan(const sg_grid_desc* desc, hipStream_t s, bool want_adjoint, Plan*
     pl->adj.xcd_map = (xm && *xm) ? std::atoi(xm) : default_xcd_map(*pl);
   }
   pl->stream_ok = ok;
-  if (ok && pl->adj.n_tiles1 > 0) {
-    Tiling& tl = pl->adj;
-    // window [lo, hi] of every block along dimension d, then per-k lists
-    auto lists = [&](int d, int nblocks, int block_len, int m, int n, int p,
-                     const std::vector<int32_t>& span, std::vector<int32_t>& ptr,
-                     std::vector<int32_t>& ent) {
-      std::vector<std::vector<int32_t>> per(n);
-      for (int b = 0; b < nblocks; ++b) {
-        const int a = b * block_len, e = std::min(a + block_len, m) - 1;
-        const int lo = span[a] - p;
-        for (int k = lo; k <= span[e]; ++k) {
-          per[k].push_back(b);
-          per[k].push_back(k - lo);
-        }
-      }
-      ptr.assign(n + 1, 0);
-      ent.clear();
-      for (int k = 0; k < n; ++k) {
-        ent.insert(ent.end(), per[k].begin(), per[k].end());
-        ptr[k + 1] = static_cast<int32_t>(ent.size() / 2);
-      }
-      (void)d;
-    };
-    std::vector<int32_t> ptr[3], ent[3];
-    const DimDev& d1 = pl->dims[0];
-    const DimDev& d2 = pl->dims[1];
-    lists(0, tl.n_tiles1, tl.tile_width(), d1.m, d1.n, d1.p, d1.h_span, ptr[0], ent[0]);
-    lists(1, tl.n_chunks2, tl.chunk_rows(), d2.m, d2.n, d2.p, d2.h_span, ptr[1], ent[1]);
-    if (pl->nin == 3) {
-      const DimDev& d3 = pl->dims[2];
-      lists(2, tl.n_z, tl.j3_per_block, d3.m, d3.n, d3.p, d3.h_span, ptr[2], ent[2]);
-    } else {
-      ptr[2] = {0, 1};
-      ent[2] = {0, 0};
-    }
+  const char* adj_mma_env = std::getenv("SG_ADJ_MMA");  // "0": streaming kernel only
+  if (ok && !pl->nurbs && want_adjoint && !(adj_mma_env && std::strcmp(adj_mma_env, "0") == 0) &&
+      adjoint_mma_shape_ok(*pl))
+    plan_adjoint_mma(*pl);
+  if (ok && (pl->adj.n_tiles1 > 0 || pl->adjm.n_tiles1 > 0)) {
     hipError_t e = hipSuccess;
-    for (int d = 0; d < 3 && e == hipSuccess; ++d) {
-      e = dalloc(*pl, &tl.gptr[d], ptr[d].size());
-      if (e == hipSuccess) e = dalloc(*pl, &tl.gent[d], ent[d].size());
+    if (pl->adj.n_tiles1 > 0) e = upload_gather_lists(*pl, pl->adj);
+    if (e == hipSuccess && pl->adjm.n_tiles1 > 0) e = upload_gather_lists(*pl, pl->adjm);
+    if (e == hipSuccess && pl->adjm.n_tiles1 > 0 && pl->nin == 2) {
+      const float one = 1.f;
+      const int32_t zero = 0;
+      e = dalloc(*pl, &pl->unit_tab, 1);
+      if (e == hipSuccess) e = dalloc(*pl, &pl->unit_span, 1);
+      if (e == hipSuccess) e = hipMemcpy(pl->unit_tab, &one, sizeof(one), hipMemcpyHostToDevice);
       if (e == hipSuccess)
-        e = hipMemcpy(tl.gptr[d], ptr[d].data(), ptr[d].size() * sizeof(int32_t),
-                      hipMemcpyHostToDevice);
-      if (e == hipSuccess && !ent[d].empty())
-        e = hipMemcpy(tl.gent[d], ent[d].data(), ent[d].size() * sizeof(int32_t),
-                      hipMemcpyHostToDevice);
+        e = hipMemcpy(pl->unit_span, &zero, sizeof(zero), hipMemcpyHostToDevice);
     }
-    pl->work_bytes = adjoint_work_bytes(*pl);
+    pl->work_bytes = std::max(adjoint_work_bytes(*pl), adjoint_mma_work_bytes(*pl));
     unsigned char* w = nullptr;
     if (e == hipSuccess) e = dalloc(*pl, &w, pl->work_bytes);
     pl->work = w;
@@ -511,6 +559,35 @@ int sg_plan_basis(const sg_plan* plan, int32_t dim, double* table, int32_t* samp
 int sg_plan_device_bytes(const sg_plan* plan, int64_t* bytes) {
   if (!plan || !bytes) return fail(SG_ERR_INVALID, "NULL argument");
   *bytes = static_cast<int64_t>(plan->p.device_bytes);
+  return SG_OK;
+}
+
+int sg_plan_describe(const sg_plan* plan, char* text, int32_t capacity) {
+  if (!plan || !text || capacity < 1) return fail(SG_ERR_INVALID, "NULL argument");
+  const Plan& pl = plan->p;
+  std::ostringstream os;
+  auto tiling = [&](const char* name, const Tiling& t) {
+    os << name << "=";
+    if (t.n_tiles1 == 0) {
+      os << "none ";
+      return;
+    }
+    os << t.tile_width() << "x" << t.chunk_rows() << "x" << t.j3_per_block << ",tiles="
+       << t.n_tiles1 << "x" << t.n_chunks2 << "x" << t.n_z << ",win=" << t.w1max << "x"
+       << t.w2max << "x" << t.w3max << " ";
+  };
+  os << "forward=" << (pl.stream_ok ? "stream" : "direct") << " ";
+  tiling("fwd", pl.fwd);
+  tiling("mma", pl.mma);
+  const char* slow = (pl.stream_ok && pl.adj.n_tiles1 > 0) ? "stream" : "csr";
+  os << "adjoint=" << (pl.adjm.n_tiles1 > 0 ? "mma_f32+" : "") << slow << " ";
+  tiling("adj", pl.adj);
+  tiling("adjm", pl.adjm);
+  os << "work_bytes=" << pl.work_bytes;
+  const std::string s = os.str();
+  const size_t n = std::min(s.size(), static_cast<size_t>(capacity - 1));
+  std::memcpy(text, s.data(), n);
+  text[n] = '\0';
   return SG_OK;
 }
 

@@ -89,8 +89,10 @@ struct Tiling {
   // offset-in-window) pairs of the workgroups whose windows hold k
   int32_t* gptr[3] = {nullptr, nullptr, nullptr};  // [n_d + 1]
   int32_t* gent[3] = {nullptr, nullptr, nullptr};  // [2 * entries]
-  int tile_width() const { return kLanes * wx; }
-  int chunk_rows() const { return rows_per_wave * (kWaves / wx); }
+  // explicit tile width / chunk height (0: derived from wx, rows_per_wave)
+  int tile_w = 0, chunk_h = 0;
+  int tile_width() const { return tile_w ? tile_w : kLanes * wx; }
+  int chunk_rows() const { return chunk_h ? chunk_h : rows_per_wave * (kWaves / wx); }
 };
 
 struct Plan {
@@ -98,8 +100,11 @@ struct Plan {
   std::vector<DimDev> dims;
   bool stream_ok = false;     // streaming (sum-factorised) kernels apply
   Tiling fwd, adj, mma;
+  Tiling adjm;                // FP32 matrix-core adjoint (n_tiles1 == 0: not used)
+  float* unit_tab = nullptr;  // {1.0f} and {0}: dimension 3 of a 2-D grid for
+  int32_t* unit_span = nullptr;  //   the matrix-core adjoint
   int nt_stores = 0;          // non-temporal stores of eval
-  void* work = nullptr;       // adjoint partial slabs
+  void* work = nullptr;       // adjoint partial slabs (either adjoint path)
   size_t work_bytes = 0;
   size_t device_bytes = 0;
   std::vector<void*> allocations;
@@ -177,5 +182,17 @@ hipError_t launch_adjoint(const Plan& plan, const int32_t* orders,
                           int accumulate, hipStream_t s);
 size_t adjoint_work_bytes(const Plan& plan);
 size_t adjoint_stream_lds(const Plan& plan, int nc, size_t elem);
+// Sum the partial windows [unit][w3max][nout][w2][w1] of tiling tl into
+// Pbar (overwrite or accumulate), one thread per control point.
+hipError_t launch_adjoint_gather(const Plan& plan, const Tiling& tl, int w1, int w2,
+                                 const void* part, void* Pbar, int dtype, int accumulate,
+                                 hipStream_t s);
+
+// ---- sg_adjoint_mma.hip ----
+bool adjoint_mma_shape_ok(const Plan& plan);
+int adjoint_mma_resident_waves(const Plan& plan);  // on the current device
+size_t adjoint_mma_work_bytes(const Plan& plan);
+hipError_t launch_adjoint_mma(const Plan& plan, const int32_t* orders, const float* ebar,
+                              float* Pbar, int accumulate, hipStream_t s);
 
 }  // namespace sg

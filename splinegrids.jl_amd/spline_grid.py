@@ -63,6 +63,12 @@ class Plan:
         _lib.check(self._lib.sg_plan_device_bytes(self.handle, ctypes.byref(out)))
         return out.value
 
+    def describe(self) -> str:
+        """The kernels and tile shapes this plan uses, as one line of text."""
+        buf = ctypes.create_string_buffer(1024)
+        _lib.check(self._lib.sg_plan_describe(self.handle, buf, len(buf)))
+        return buf.value.decode()
+
     def __del__(self):
         h = getattr(self, "handle", None)
         if h is not None and h.value:
