@@ -82,11 +82,13 @@ void compute_tiling(const Plan& pl, Tiling& t, int rows_per_wave, int wx = 1) {
   }
 }
 
+// Largest LDS claim of a forward launch (FP64, widest channel block): the
+// band window of fwd_stream; the MFMA kernels add their chunk's B2 rows.
 size_t forward_lds_worst(const Plan& pl, const Tiling& t) {
   const int nc = std::min(pl.nout, 4) + (pl.nurbs ? 1 : 0);
   const size_t p2 = static_cast<size_t>(pl.dims[1].p);
-  return (static_cast<size_t>(nc) * t.w1max * t.w2max +
-          static_cast<size_t>(SG_MAX_FIELDS) * t.chunk_rows() * (p2 + 1)) * sizeof(double) +
+  return (static_cast<size_t>(nc) * t.w1max * t.w2max + t.chunk_rows() * (p2 + 1)) *
+             sizeof(double) +
          static_cast<size_t>(t.chunk_rows()) * sizeof(int);
 }
 
@@ -428,10 +430,12 @@ int build_plan(const sg_grid_desc* desc, hipStream_t s, bool want_adjoint, Plan*
   if (ok) {
     const char* xm = std::getenv("SG_XCD_MAP");
     const int xcd = (xm && *xm) ? std::atoi(xm) : default_xcd_map(*pl);
-    // forward: waves side by side along dimension 1 when it is long enough
-    int wx = env_int("SG_FWD_WX", pl->dims[0].m >= 1024 ? 4 : 1);
+    // forward: one wave per 64 samples of dimension 1; 32 rows per wave when
+    // dimension 2 splits into whole chunks of 128 rows, else 16 (a ragged
+    // last chunk of 32-row waves leaves most of its workgroup idle)
+    int wx = env_int("SG_FWD_WX", 1);
     if (wx != 1 && wx != 2 && wx != 4) wx = 1;
-    int rows = env_int("SG_FWD_ROWS", 32);
+    int rows = env_int("SG_FWD_ROWS", pl->dims[1].m % (32 * kWaves) == 0 ? 32 : 16);
     compute_tiling(*pl, pl->fwd, rows, wx);
     while (forward_lds_worst(*pl, pl->fwd) > static_cast<size_t>(kMaxLds) &&
            (rows > 1 || wx > 1)) {

@@ -32,30 +32,43 @@ namespace {
 template <typename T>
 struct StreamArgs {
   BandSrc<T> src;                // control points of the tile (stage 1)
-  T* out[SG_MAX_FIELDS];
   const T* tab1[SG_MAX_FIELDS];  // dimension 1 at each field's order [m1][p1+1]
-  const T* tab2[SG_MAX_FIELDS];  // dimension 2 [m2][P2+1]
   const int32_t* span1;
-  const int32_t* span2;
   int m1, m2;
   int p1;
   int c0;                        // first channel handled by this launch
-  int nf;                        // fields actually stored (<= NF)
   int rows_per_wave;
   int wx;                        // waves side by side along dimension 1
   int ntx, nty;                  // tiles along dimension 1, chunks along 2
   int xcd_map;                   // workgroup numbering mode
   int nt;                        // non-temporal stores of eval
-  int gmax;                      // LDS elements reserved per channel of G
   long long nsamp;               // elements per channel of eval
 };
 
+// Per-row data of the row loop: dimension 2's spans, the B2 table of every
+// field, and the output of every field.  They are kernel parameters of their
+// own, declared __restrict__, so that the compiler may read the row-uniform
+// spans and B2 rows with scalar loads (the scalar cache serves the whole
+// wave, no vector registers or lane broadcasts are spent on them, and scalar
+// loads do not share the vector-memory counter with the stores of eval).
+template <typename T>
+struct RowPtrs {
+  const int32_t* span2;
+  const T* tab2[SG_MAX_FIELDS];
+  T* out[SG_MAX_FIELDS];
+};
+
 template <typename T, int NIN, int P2, int NF, int NC, bool NURBS>
-__global__ __launch_bounds__(kThreads) void fwd_stream(StreamArgs<T> a) {
+__global__ __launch_bounds__(kThreads) void fwd_stream(
+    StreamArgs<T> a, const int32_t* __restrict__ span2, const T* __restrict__ tb0,
+    const T* __restrict__ tb1, const T* __restrict__ tb2, const T* __restrict__ tb3,
+    T* __restrict__ ou0, T* __restrict__ ou1, T* __restrict__ ou2, T* __restrict__ ou3) {
   extern __shared__ __align__(16) unsigned char smem[];
   T* G = reinterpret_cast<T*>(smem);
   constexpr int NCH = NC + (NURBS ? 1 : 0);
   constexpr int PM = kMaxStreamDegree;
+  const T* const tab2[4] = {tb0, tb1, tb2, tb3};
+  T* const outs[4] = {ou0, ou1, ou2, ou3};
 
   const int lane = threadIdx.x & (kLanes - 1);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kLanes);
@@ -65,37 +78,21 @@ __global__ __launch_bounds__(kThreads) void fwd_stream(StreamArgs<T> a) {
   const int j1_0 = tile.j1_0, r0 = tile.r0, r1 = tile.r1, j3 = tile.j3;
   const int lo1 = a.span1[j1_0] - a.p1;
   const int W1 = a.span1[tile.j1_last] - lo1 + 1;
-  const int lo2 = a.span2[r0] - P2;
-  const int W2 = a.span2[r1 - 1] - lo2 + 1;
+  const int lo2 = span2[r0] - P2;
+  const int W2 = span2[r1 - 1] - lo2 + 1;
   const int cs = W1 * W2;  // LDS elements per channel
-  // Row data of the chunk in LDS: spans and B2 of every field.  Reading them
-  // from LDS keeps vector-memory loads (which share vmcnt with the stores of
-  // eval) out of the row loop, so no row waits for earlier stores.
-  const int chunk = r1 - r0;
-  const int chunk_cap = a.rows_per_wave * (kWaves / a.wx);
-  T* RT = G + static_cast<size_t>(NCH) * a.gmax;                  // [NF][chunk][P2+1]
-  int* RS = reinterpret_cast<int*>(RT + static_cast<size_t>(NF) * chunk_cap * (P2 + 1));
 
   // ---- stage 1 ----
   stage1_band_window<T, NIN, NC, NURBS>(G, a.src, j3, lo1, W1, lo2, W2);
-  {
-    const size_t rbase = static_cast<size_t>(r0) * (P2 + 1);
-    const size_t fstride = static_cast<size_t>(chunk_cap) * (P2 + 1);
-    stage_to_lds<1, 1>(chunk, [&](int, int i) { return a.span2[r0 + i]; },
-                       [&](int, int i, int v) { RS[i] = v; });
-    stage_to_lds<NF, 4>(chunk * (P2 + 1), [&](int f, int i) { return a.tab2[f][rbase + i]; },
-                        [&](int f, int i, T v) { RT[f * fstride + i] = v; });
-  }
   __syncthreads();
 
   // ---- stage 2 ----
   const int wave_x = wave % a.wx, wave_y = wave / a.wx;
   const int rb = r0 + wave_y * a.rows_per_wave;
   const int re = min(rb + a.rows_per_wave, r1);
-  if (rb >= re) return;  // no barrier follows
-  const int j1w = j1_0 + kLanes * wave_x + lane;
-  const bool active = j1w < a.m1;
-  const int j1 = active ? j1w : a.m1 - 1;
+  const int j1 = j1_0 + kLanes * wave_x + lane;
+  // no barrier follows: lanes past the grid, and waves without rows, leave
+  if (rb >= re || j1 >= a.m1) return;
   const int p1 = a.p1;
   const int q1b = a.span1[j1] - p1 - lo1;
 
@@ -125,15 +122,33 @@ __global__ __launch_bounds__(kThreads) void fwd_stream(StreamArgs<T> a) {
       }
   };
 
-  int cur = wave_uniform(RS[rb - r0]);
+  int cur = span2[rb];
 #pragma unroll
   for (int q = 0; q <= P2; ++q) column(cur - P2 + q, V[q]);
+
+  // span and B2 rows of row j2 (scalar loads), fetched one row ahead
+  auto row_data = [&](int j2, int& s2, T (&bb)[NF][P2 + 1]) {
+    s2 = span2[j2];
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+#pragma unroll
+      for (int q = 0; q <= P2; ++q) bb[f][q] = tab2[f][static_cast<size_t>(j2) * (P2 + 1) + q];
+  };
+  int s_next;
+  T b_next[NF][P2 + 1];
+  row_data(rb, s_next, b_next);
 
   const long long obase = static_cast<long long>(j1) +
                           static_cast<long long>(a.m1) * a.m2 * j3 +
                           a.nsamp * a.c0;
   for (int j2 = rb; j2 < re; ++j2) {
-    const int s2 = wave_uniform(RS[j2 - r0]);
+    const int s2 = s_next;
+    T bb[NF][P2 + 1];
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+#pragma unroll
+      for (int q = 0; q <= P2; ++q) bb[f][q] = b_next[f][q];
+    row_data(min(j2 + 1, re - 1), s_next, b_next);
     if (s2 != cur) {
       if (s2 - cur > P2) {
         cur = s2;
@@ -153,17 +168,8 @@ __global__ __launch_bounds__(kThreads) void fwd_stream(StreamArgs<T> a) {
       }
     }
     const long long orow = obase + static_cast<long long>(a.m1) * j2;
-    // B2 rows of every field first (slots of fields >= nf hold rows of
-    // field 0, see the launcher), so one LDS wait covers the row
-    T bb[NF][P2 + 1];
-#pragma unroll
-    for (int f = 0; f < NF; ++f)
-#pragma unroll
-      for (int q = 0; q <= P2; ++q)
-        bb[f][q] = wave_uniform(RT[(static_cast<size_t>(f) * chunk_cap + (j2 - r0)) * (P2 + 1) + q]);
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
-      if (f >= a.nf) break;
       T num[NCH];
 #pragma unroll
       for (int ch = 0; ch < NCH; ++ch) {
@@ -172,16 +178,14 @@ __global__ __launch_bounds__(kThreads) void fwd_stream(StreamArgs<T> a) {
         for (int q = 0; q <= P2; ++q) s += bb[f][q] * V[q][f][ch];
         num[ch] = s;
       }
-      if (active) {
 #pragma unroll
-        for (int c = 0; c < NC; ++c) {
-          const T v = NURBS ? num[c] / num[NCH - 1] : num[c];
-          T* dst = a.out[f] + orow + a.nsamp * c;
-          if (a.nt)
-            __builtin_nontemporal_store(v, dst);
-          else
-            *dst = v;
-        }
+      for (int c = 0; c < NC; ++c) {
+        const T v = NURBS ? num[c] / num[NCH - 1] : num[c];
+        T* dst = outs[f] + orow + a.nsamp * c;
+        if (a.nt)
+          __builtin_nontemporal_store(v, dst);
+        else
+          *dst = v;
       }
     }
   }
@@ -262,7 +266,8 @@ __global__ __launch_bounds__(256) void fwd_direct(DirectArgs<T> a) {
 // ---- kernel selection ----
 
 template <typename T>
-using StreamKernel = void (*)(StreamArgs<T>);
+using StreamKernel = void (*)(StreamArgs<T>, const int32_t*, const T*, const T*, const T*,
+                              const T*, T*, T*, T*, T*);
 
 template <typename T, int NIN, int P2>
 StreamKernel<T> pick_variant(int nf, int nc, bool nurbs) {
@@ -286,7 +291,8 @@ StreamKernel<T> pick_variant(int nf, int nc, bool nurbs) {
   }
   if (nc != 1) return nullptr;
   if (nf == 2) return fwd_stream<T, NIN, P2, 2, 1, false>;
-  if (nf <= 4) return fwd_stream<T, NIN, P2, 4, 1, false>;
+  if (nf == 3) return fwd_stream<T, NIN, P2, 3, 1, false>;
+  if (nf == 4) return fwd_stream<T, NIN, P2, 4, 1, false>;
   return nullptr;
 }
 
@@ -316,14 +322,10 @@ const double* ktab<double>(const DimDev& d, int order) {
 
 }  // namespace
 
-// LDS of fwd_stream: G for nc (+1 weight) channels, then the chunk's B2 rows
-// of nf fields and its spans.
-size_t fwd_stream_lds(const Plan& pl, int nc, int nf, size_t elem) {
+// LDS of fwd_stream: the band window G of nc (+1 weight) channels.
+size_t fwd_stream_lds(const Plan& pl, int nc, size_t elem) {
   const Tiling& t = pl.fwd;
-  const size_t p2 = static_cast<size_t>(pl.dims[1].p);
-  return (static_cast<size_t>(nc + (pl.nurbs ? 1 : 0)) * t.w1max * t.w2max +
-          static_cast<size_t>(nf) * t.chunk_rows() * (p2 + 1)) * elem +
-         static_cast<size_t>(t.chunk_rows()) * sizeof(int);
+  return static_cast<size_t>(nc + (pl.nurbs ? 1 : 0)) * t.w1max * t.w2max * elem;
 }
 
 namespace {
@@ -342,13 +344,11 @@ hipError_t forward_stream(const Plan& pl, int nf, const int32_t* orders,
   a.src.n1 = d1.n;
   a.src.n2 = d2.n;
   a.span1 = d1.span;
-  a.span2 = d2.span;
   a.m1 = d1.m;
   a.m2 = d2.m;
   a.p1 = d1.p;
   a.rows_per_wave = pl.fwd.rows_per_wave;
   a.wx = pl.fwd.wx;
-  a.gmax = pl.fwd.w1max * pl.fwd.w2max;
   a.ntx = pl.fwd.n_tiles1;
   a.nty = pl.fwd.n_chunks2;
   a.xcd_map = pl.fwd.xcd_map;
@@ -382,26 +382,27 @@ hipError_t forward_stream(const Plan& pl, int nf, const int32_t* orders,
       if (nurbs) break;
     }
     const int gnf = static_cast<int>(group.size());
-    for (int g = 0; g < SG_MAX_FIELDS; ++g) {
-      const int f = group[g < gnf ? g : 0];
+    const T* tab2[SG_MAX_FIELDS] = {nullptr, nullptr, nullptr, nullptr};
+    T* out[SG_MAX_FIELDS] = {nullptr, nullptr, nullptr, nullptr};
+    for (int g = 0; g < gnf; ++g) {
+      const int f = group[g];
       a.tab1[g] = ktab<T>(d1, orders[f * nin + 0]);
-      a.tab2[g] = ktab<T>(d2, orders[f * nin + 1]);
-      a.out[g] = outs[f];
+      tab2[g] = ktab<T>(d2, orders[f * nin + 1]);
+      out[g] = outs[f];
     }
     if (nin == 3) a.src.tab3 = ktab<T>(pl.dims[2], orders[group[0] * nin + 2]);
-    a.nf = gnf;
     int c0 = 0;
     while (c0 < pl.nout) {
       const int left = pl.nout - c0;
       const int nc = (gnf > 1) ? 1 : (left >= 4 ? 4 : left);
-      const int kf = gnf == 1 ? 1 : (gnf == 2 ? 2 : 4);
-      StreamKernel<T> k = (nin == 2) ? pick_stream<T, 2>(d2.p, kf, nc, nurbs)
-                                     : pick_stream<T, 3>(d2.p, kf, nc, nurbs);
+      StreamKernel<T> k = (nin == 2) ? pick_stream<T, 2>(d2.p, gnf, nc, nurbs)
+                                     : pick_stream<T, 3>(d2.p, gnf, nc, nurbs);
       if (!k) return hipErrorInvalidValue;
       a.c0 = c0;
       a.src.c0 = c0;
-      const size_t lds = fwd_stream_lds(pl, nc, kf, sizeof(T));
-      hipLaunchKernelGGL(k, grid, dim3(kThreads), lds, s, a);
+      const size_t lds = fwd_stream_lds(pl, nc, sizeof(T));
+      hipLaunchKernelGGL(k, grid, dim3(kThreads), lds, s, a, d2.span, tab2[0], tab2[1],
+                         tab2[2], tab2[3], out[0], out[1], out[2], out[3]);
       hipError_t e = hipGetLastError();
       if (e != hipSuccess) return e;
       c0 += nc;

@@ -164,7 +164,7 @@ hipError_t launch_copy_f64(const double* in, size_t count, double* out,
                            hipStream_t s);
 
 // ---- sg_forward.hip ----
-size_t fwd_stream_lds(const Plan& plan, int nc, int nf, size_t elem);
+size_t fwd_stream_lds(const Plan& plan, int nc, size_t elem);
 // Fields: derivative orders [nf][nin]; one output pointer per field.
 hipError_t launch_forward(const Plan& plan, int nf, const int32_t* orders,
                           const void* P, const void* W, void* const* outs,
