@@ -202,8 +202,9 @@ def cpu_baseline(wl, budget_s: float = 10.0):
 
 
 def sampled_parity(sg, grid, wl, outs, n_points: int = 20000):
-    """Max |GPU - FP64 oracle| over random samples of each field, relative to
-    max|field| on the GPU (the oracle consumes the same Tv-rounded inputs)."""
+    """Max and RMS |GPU - FP64 oracle| over random samples of each field,
+    relative to max|field| on the GPU (the oracle consumes the same
+    Tv-rounded inputs).  Returns (max per field, RMS per field)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
@@ -214,7 +215,7 @@ def sampled_parity(sg, grid, wl, outs, n_points: int = 20000):
     lin = rng.integers(0, nsamp, n_points)
     P = wl["P"].astype(wl["dtype"]).astype(np.float64)
     W = None if wl["W"] is None else wl["W"].astype(wl["dtype"]).astype(np.float64)
-    errs = []
+    errs, rms = [], []
     for f, out in zip(wl["fields"], outs):
         want = O.eval_points(dims, P, lin, deriv=f, W=W, nthreads=max(1, min(16, os.cpu_count() or 1)))
         flat = sg.vec_colmajor(out).view(-1, 1) if wl["nout"] == 1 else \
@@ -222,7 +223,8 @@ def sampled_parity(sg, grid, wl, outs, n_points: int = 20000):
         got = flat[lin].double().cpu().numpy()
         scale = float(out.abs().max())
         errs.append(float(np.abs(got - want).max()) / scale)
-    return errs
+        rms.append(float(np.sqrt(np.mean((got - want) ** 2))) / scale)
+    return errs, rms
 
 
 def traffic_from_profiles(workload_name: str):
@@ -283,6 +285,26 @@ def settle_gc() -> None:
     launches of a short kernel."""
     gc.collect()
     gc.freeze()
+
+
+def timed_steps(step, steps: int, warmup: int):
+    """Average ms per step over `steps` back-to-back steps on the current
+    stream (HIP events), after `warmup` untimed steps; also the wall time."""
+    import torch
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    settle_gc()
+    stream = torch.cuda.current_stream()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t_wall = time.perf_counter()
+    start.record(stream)
+    for _ in range(steps):
+        step()
+    stop.record(stream)
+    torch.cuda.synchronize()
+    return start.elapsed_time(stop) / steps, time.perf_counter() - t_wall
 
 
 def bench_batch(args, sg, wl):
@@ -353,6 +375,8 @@ def main(argv=None):
     ap.add_argument("--tier", default="native")
     ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-tiers", action="store_true",
+                    help="skip timing the split-FP16 and BF16 tiers beside FP32")
     ap.add_argument("--separate-fields", action="store_true",
                     help="one launch per field instead of the fused launch")
     args = ap.parse_args(argv)
@@ -380,31 +404,20 @@ def main(argv=None):
         ebar = sg.colmajor(torch.randn(grid.eval.shape, dtype=grid.dtype, device="cuda"))
         pbar = sg.colmajor_zeros(grid.control_points.shape, grid.dtype, "cuda")
 
-    def step():
-        if adjoint:
-            sg.evaluate_adjoint_(grid, eval=ebar, control_points=pbar, tier=args.tier)
-        elif len(fields) == 1:
-            sg.evaluate_(grid, derivative_order=fields[0], tier=args.tier)
-        elif args.separate_fields:
-            for f, o in zip(fields, outs):
-                sg.evaluate_(grid, derivative_order=f, eval=o, tier=args.tier)
-        else:
-            sg.evaluate_fields_(grid, fields, outs, tier=args.tier)
+    def make_step(tier):
+        def step():
+            if adjoint:
+                sg.evaluate_adjoint_(grid, eval=ebar, control_points=pbar, tier=tier)
+            elif len(fields) == 1:
+                sg.evaluate_(grid, derivative_order=fields[0], tier=tier)
+            elif args.separate_fields:
+                for f, o in zip(fields, outs):
+                    sg.evaluate_(grid, derivative_order=f, eval=o, tier=tier)
+            else:
+                sg.evaluate_fields_(grid, fields, outs, tier=tier)
+        return step
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    settle_gc()
-    stream = torch.cuda.current_stream()
-    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    t_wall = time.perf_counter()
-    start.record(stream)
-    for _ in range(args.steps):
-        step()
-    stop.record(stream)
-    torch.cuda.synchronize()
-    t_wall = time.perf_counter() - t_wall
-    ms = start.elapsed_time(stop) / args.steps
+    ms, t_wall = timed_steps(make_step(args.tier), args.steps, args.warmup)
     nsamp = math.prod(wl["m"])
     n_fields = 1 if adjoint else len(fields)
     value = n_fields * nsamp / (ms * 1e-3)
@@ -433,7 +446,29 @@ def main(argv=None):
         "wall_s_timed": t_wall,
     }
     if not adjoint:
-        out["max_rel_err_vs_fp64"] = sampled_parity(sg, grid, wl, outs)
+        out["max_rel_err_vs_fp64"], out["rms_rel_err_vs_fp64"] = sampled_parity(sg, grid, wl, outs)
+        # the other arithmetic tiers on the same inputs, timed the same way
+        if wl["dtype"] == np.float32 and args.tier == "native" and not args.no_tiers:
+            tiers = {"native": {"ms_per_step": ms, "value": value,
+                                "max_rel_err_vs_fp64": out["max_rel_err_vs_fp64"],
+                                "rms_rel_err_vs_fp64": out["rms_rel_err_vs_fp64"]}}
+            for tier in ("split_f16", "bf16"):
+                try:
+                    t_ms, _ = timed_steps(make_step(tier), max(20, args.steps // 2), args.warmup)
+                except sg.SplineGridsError as e:
+                    tiers[tier] = {"unsupported": str(e)}
+                    continue
+                mx, rms = sampled_parity(sg, grid, wl, outs)
+                tiers[tier] = {"ms_per_step": t_ms, "value": n_fields * nsamp / (t_ms * 1e-3),
+                               "max_rel_err_vs_fp64": mx, "rms_rel_err_vs_fp64": rms}
+            out["tiers"] = tiers
+    else:
+        # forward (evaluate!) plus backward (evaluate_adjoint!) per step
+        def fwd_bwd():
+            sg.evaluate_(grid)
+            sg.evaluate_adjoint_(grid, eval=ebar, control_points=pbar)
+        out["forward_plus_adjoint_ms"], _ = timed_steps(fwd_bwd, max(20, args.steps // 2),
+                                                        args.warmup)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(wl, args.cpu_budget)
         out["cpu_baseline"]["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
