@@ -354,6 +354,7 @@ def bench_batch(args, sg, wl):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded inputs of the workload's shape)",
         "config": {"workload": wl["name"], "grids": len(grids), **info,
+                   "describe": batch.describe(),
                    "setup_s": setup_s},
         "roofline": {"bound": "hbm", "achieved": B / (ms_graph * 1e-3) / 1e9,
                      "peak": PEAK_HBM_GBS, "unit": "GB/s",

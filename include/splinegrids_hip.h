@@ -172,8 +172,10 @@ int sg_batch_evaluate(sg_batch* batch, int32_t tier, void* stream);
  * launch per grid (native tier) on `stream`. */
 int sg_batch_evaluate_each(sg_batch* batch, void* stream);
 
-/* Time candidate tilings (rows per wave, persistent workgroups per CU) on
- * the arrays set last and keep the fastest; `reps` timed launches each. */
+/* Time candidate tilings (workgroups per CU {2,3,4,6} x rows per wave
+ * {8,16,32}; each grid's rows are halved until its tiles fit the LDS share of
+ * one workgroup) on the arrays set last and keep the fastest; `reps` timed
+ * launches each. */
 int sg_batch_autotune(sg_batch* batch, int32_t reps, void* stream);
 
 /* Totals of the batch: tiles, sample points, grids evaluated by the
@@ -181,6 +183,11 @@ int sg_batch_autotune(sg_batch* batch, int32_t reps, void* stream);
 int sg_batch_info(const sg_batch* batch, int64_t* n_tiles, int64_t* n_samples,
                   int32_t* n_persistent, int32_t* rows_per_wave,
                   int32_t* blocks_per_cu);
+
+/* One line of text with the batch's tiling, LDS per workgroup and resident
+ * workgroups per CU (for logs and tests), truncated to capacity-1 characters
+ * and NUL-terminated. */
+int sg_batch_describe(const sg_batch* batch, char* text, int32_t capacity);
 
 #ifdef __cplusplus
 }

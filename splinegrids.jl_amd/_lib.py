@@ -23,7 +23,7 @@ EXPORTS = (
     "sg_plan_destroy", "sg_plan_basis", "sg_plan_device_bytes", "sg_plan_describe", "sg_evaluate",
     "sg_evaluate_fields", "sg_evaluate_adjoint", "sg_stream_synchronize",
     "sg_batch_create", "sg_batch_destroy", "sg_batch_set_arrays", "sg_batch_evaluate",
-    "sg_batch_evaluate_each", "sg_batch_autotune", "sg_batch_info",
+    "sg_batch_evaluate_each", "sg_batch_autotune", "sg_batch_info", "sg_batch_describe",
 )
 
 
@@ -91,6 +91,7 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
         "sg_batch_autotune": (ctypes.c_int, [vp, i32, vp]),
         "sg_batch_info": (ctypes.c_int, [vp, ctypes.POINTER(i64), ctypes.POINTER(i64), i32p,
                                          i32p, i32p]),
+        "sg_batch_describe": (ctypes.c_int, [vp, ctypes.c_char_p, i32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

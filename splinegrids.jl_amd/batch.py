@@ -98,6 +98,12 @@ class SplineGridBatch:
                 "grids": len(self.grids), "rows_per_wave": rows.value,
                 "blocks_per_cu": bpc.value}
 
+    def describe(self) -> str:
+        """Tiling, LDS per workgroup and resident workgroups per CU, one line."""
+        buf = ctypes.create_string_buffer(1024)
+        _lib.check(self._lib.sg_batch_describe(self.handle, buf, len(buf)))
+        return buf.value.decode()
+
     def __del__(self):
         h = getattr(self, "handle", None)
         if h is not None and h.value:

@@ -11,6 +11,8 @@
 // selected by a switch over p2.
 #include <algorithm>
 #include <numeric>
+#include <cstring>
+#include <sstream>
 #include <vector>
 
 #include "sg_tiles.hpp"
@@ -36,7 +38,7 @@ struct BGrid {
 };
 
 struct BTile {
-  int grid, bx, by, j3;
+  int grid, bx, by, bz;  // bz: samples j3 from kWaves * bz (3-D grids)
 };
 
 namespace {
@@ -50,50 +52,66 @@ __global__ void batch_reset(int* counter) {
 template <int P2>
 __device__ void batch_tile(const BGrid& g, const BTile& t, float* smem, int lane, int wave) {
   constexpr int PM = kMaxStreamDegree;
+  // the grid's arrays as global pointers (global loads and stores)
+  const auto span1 = global_ptr(g.span1), span2 = global_ptr(g.span2), span3 = global_ptr(g.span3);
+  const auto tab1 = global_ptr(g.tab1), tab2 = global_ptr(g.tab2), tab3 = global_ptr(g.tab3);
+  const auto Pg = global_ptr(g.P), Wg = global_ptr(g.W);
+  const auto outg = global_ptr(g.out);
   const int chunk_cap = g.rows_per_wave * kWaves;
   const int j1_0 = t.bx * kLanes;
   const int j1_last = min(j1_0 + kLanes - 1, g.m1 - 1);
   const int r0 = t.by * chunk_cap, r1 = min(r0 + chunk_cap, g.m2);
   const int p1 = g.p1;
-  const int lo1 = g.span1[j1_0] - p1;
-  const int W1 = g.span1[j1_last] - lo1 + 1;
-  const int lo2 = g.span2[r0] - P2;
-  const int W2 = g.span2[r1 - 1] - lo2 + 1;
+  const int lo1 = span1[j1_0] - p1;
+  const int W1 = span1[j1_last] - lo1 + 1;
+  const int lo2 = span2[r0] - P2;
+  const int W2 = span2[r1 - 1] - lo2 + 1;
   const int cs = W1 * W2;
   const bool nurbs = g.nurbs != 0;
   const int nch = min(kBC, g.nout) + (nurbs ? 1 : 0);
-  // [nch][cs] window slabs (weight last), then the chunk's B2 rows and spans
-  float* G = smem;
-  float* RT = smem + static_cast<size_t>(nch) * cs;  // [chunk][P2+1]
+  // 3-D grids: the tile holds kWaves consecutive samples j3 (from
+  // kWaves * t.bz), one per wave, and every wave walks all rows of the chunk;
+  // 2-D grids: one plane, the waves split the chunk's rows
+  const bool zpack = g.nin == 3;
+  const int j3_0 = zpack ? t.bz * kWaves : 0;
+  const int planes = zpack ? min(kWaves, g.m3 - j3_0) : 1;
+  // [planes][nch][cs] window slabs (weight last), then the chunk's B2 rows
+  // and spans (space for kWaves planes is reserved for 3-D grids, as the
+  // host's LDS size assumes)
+  float* RT = smem + static_cast<size_t>(zpack ? kWaves : 1) * nch * cs;  // [chunk][P2+1]
   int* RS = reinterpret_cast<int*>(RT + static_cast<size_t>(chunk_cap) * (P2 + 1));
-  const int j3 = t.j3;
 
   {
     const size_t rbase = static_cast<size_t>(r0) * (P2 + 1);
-    stage_to_lds<1, 1>(r1 - r0, [&](int, int i) { return g.span2[r0 + i]; },
+    stage_to_lds<1, 1>(r1 - r0, [&](int, int i) { return span2[r0 + i]; },
                        [&](int, int i, int v) { RS[i] = v; });
-    stage_to_lds<1, 4>((r1 - r0) * (P2 + 1), [&](int, int i) { return g.tab2[rbase + i]; },
+    stage_to_lds<1, 4>((r1 - r0) * (P2 + 1), [&](int, int i) { return tab2[rbase + i]; },
                        [&](int, int i, float v) { RT[i] = v; });
   }
 
-  const int rb = r0 + wave * g.rows_per_wave;
-  const int re = min(rb + g.rows_per_wave, r1);
+  const int rb = zpack ? (wave < planes ? r0 : r1) : r0 + wave * g.rows_per_wave;
+  const int re = zpack ? r1 : min(rb + g.rows_per_wave, r1);
+  const int j3 = j3_0 + (zpack ? wave : 0);
+  const float* G = smem + (zpack ? static_cast<size_t>(wave) * nch * cs : 0);
   const bool active = (j1_0 + lane) < g.m1;
   const int j1 = active ? j1_0 + lane : g.m1 - 1;
-  const int q1b = g.span1[j1] - p1 - lo1;
+  const int q1b = span1[j1] - p1 - lo1;
   // b1 is zero past p1, so the window sums take all PM+1 terms without a
   // branch (the clamped index re-reads the last element of the band)
   float b1[PM + 1];
 #pragma unroll
   for (int q = 0; q <= PM; ++q) {
-    const float v = g.tab1[static_cast<size_t>(j1) * (p1 + 1) + min(q, p1)];
+    const float v = tab1[static_cast<size_t>(j1) * (p1 + 1) + min(q, p1)];
     b1[q] = q <= p1 ? v : 0.f;
   }
 
   for (int c0 = 0; c0 < g.nout; c0 += kBC) {
     const int nc = min(kBC, g.nout - c0);
-    // ---- stage 1: band window, contracted along dimension 3 ----
-    for (int idx = threadIdx.x; idx < cs; idx += kThreads) {
+    // ---- stage 1: band windows, contracted along dimension 3 ----
+    for (int pidx = threadIdx.x; pidx < planes * cs; pidx += kThreads) {
+      const int pz = pidx / cs, idx = pidx - pz * cs;
+      const int jz = j3_0 + pz;
+      float* Gp = smem + static_cast<size_t>(pz) * nch * cs;
       const int q2 = idx / W1, q1 = idx - q2 * W1;
       const long long cpi =
           static_cast<long long>(lo1 + q1) + static_cast<long long>(g.n1) * (lo2 + q2);
@@ -101,29 +119,29 @@ __device__ void batch_tile(const BGrid& g, const BTile& t, float* smem, int lane
       float accw = 0.f;
       if (g.nin == 2) {
         float w = 1.f;
-        if (nurbs) w = g.W[cpi];
+        if (nurbs) w = Wg[cpi];
         accw = w;
 #pragma unroll
         for (int c = 0; c < kBC; ++c)
-          if (c < nc) acc[c] = w * g.P[cpi + g.ncp * (c0 + c)];
+          if (c < nc) acc[c] = w * Pg[cpi + g.ncp * (c0 + c)];
       } else {
-        const int s3 = g.span3[j3];
-        const float* b3 = g.tab3 + static_cast<size_t>(j3) * (g.p3 + 1);
+        const int s3 = span3[jz];
+        const auto b3 = tab3 + static_cast<size_t>(jz) * (g.p3 + 1);
         const long long plane = static_cast<long long>(g.n1) * g.n2;
         for (int d = 0; d <= g.p3; ++d) {
           const long long off = cpi + plane * (s3 - g.p3 + d);
           float bw = b3[d];
-          if (nurbs) bw *= g.W[off];
+          if (nurbs) bw *= Wg[off];
           accw += bw;
 #pragma unroll
           for (int c = 0; c < kBC; ++c)
-            if (c < nc) acc[c] += bw * g.P[off + g.ncp * (c0 + c)];
+            if (c < nc) acc[c] += bw * Pg[off + g.ncp * (c0 + c)];
         }
       }
 #pragma unroll
       for (int c = 0; c < kBC; ++c)
-        if (c < nc) G[c * cs + idx] = acc[c];
-      if (nurbs) G[nc * cs + idx] = accw;
+        if (c < nc) Gp[c * cs + idx] = acc[c];
+      if (nurbs) Gp[nc * cs + idx] = accw;
     }
     __syncthreads();
 
@@ -183,7 +201,7 @@ __device__ void batch_tile(const BGrid& g, const BTile& t, float* smem, int lane
           const long long o = obase + static_cast<long long>(g.m1) * j2;
 #pragma unroll
           for (int c = 0; c < kBC; ++c)
-            if (c < nc) g.out[o + g.nsamp * c] = nurbs ? num[c] / num[kBC] : num[c];
+            if (c < nc) outg[o + g.nsamp * c] = nurbs ? num[c] / num[kBC] : num[c];
         }
       }
     }
@@ -202,10 +220,13 @@ __global__ __launch_bounds__(kThreads) void batch_persistent(const BGrid* __rest
   for (;;) {
     if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1);
     __syncthreads();
-    const int ti = s_tile;
-    if (ti >= n_tiles) break;  // every thread read the same value: uniform exit
+    // every thread read the same value: the exit is uniform, and with the
+    // index in a scalar register the tile and grid records are scalar loads
+    // (branches on them stay scalar branches)
+    const int ti = wave_uniform(s_tile);
+    if (ti >= n_tiles) break;
     const BTile t = tiles[ti];
-    const BGrid g = grids[t.grid];
+    const BGrid g = grids[wave_uniform(t.grid)];
     switch (g.p2) {
       case 0: batch_tile<0>(g, t, smem, lane, wave); break;
       case 1: batch_tile<1>(g, t, smem, lane, wave); break;
@@ -237,7 +258,7 @@ struct sg_batch {
   int n_tiles = 0;
   size_t lds = 0;
   int rows_per_wave = 16;
-  int blocks_per_cu = 4;
+  int blocks_per_cu = 3;
   int n_cu = 0;
   int device = 0;
   long long n_samples = 0;
@@ -277,14 +298,26 @@ GridTiling grid_tiling(const Plan& pl, int rows) {
     t.w2 = std::max(t.w2, d2.h_span[e] - d2.h_span[a] + d2.p + 1);
   }
   const size_t nch = std::min(pl.nout, kBC) + (pl.nurbs ? 1 : 0);
-  t.lds = (nch * t.w1 * t.w2 + static_cast<size_t>(chunk) * (d2.p + 2)) * sizeof(float);
+  const size_t planes = pl.nin == 3 ? kWaves : 1;
+  t.lds = (planes * nch * t.w1 * t.w2 + static_cast<size_t>(chunk) * (d2.p + 2)) * sizeof(float);
   return t;
 }
 
+// LDS of one CU; a persistent launch of `bpc` workgroups per CU gives each
+// at most a bpc-th of it, or fewer workgroups than asked are resident.
+constexpr size_t kLdsPerCu = 160 * 1024;
+
+static size_t lds_cap(int blocks_per_cu) {
+  const size_t share = kLdsPerCu / static_cast<size_t>(std::max(1, blocks_per_cu));
+  return std::min(static_cast<size_t>(kMaxLds), (share & ~size_t(511)) - 512);
+}
+
 // Tiles of every persistent grid, largest first.  Each grid takes the
-// batch's rows per wave, halved until its tiles fit in LDS; a grid whose
-// tiles never fit is evaluated with its own launch instead.
-int build_tiles(sg_batch* b, int rows_per_wave) {
+// batch's rows per wave, halved until its tiles fit in the LDS share of one
+// of `blocks_per_cu` workgroups; a grid whose tiles never fit is evaluated
+// with its own launch instead.
+int build_tiles(sg_batch* b, int rows_per_wave, int blocks_per_cu) {
+  const size_t cap = lds_cap(blocks_per_cu);
   std::vector<BTile> tiles;
   std::vector<long long> cost;
   size_t lds = 16;
@@ -294,8 +327,8 @@ int build_tiles(sg_batch* b, int rows_per_wave) {
     const Plan& pl = *b->plans[gi];
     int rows = rows_per_wave;
     GridTiling t = grid_tiling(pl, rows);
-    while (t.lds > static_cast<size_t>(kMaxLds) && rows > 1) t = grid_tiling(pl, rows /= 2);
-    if (t.lds > static_cast<size_t>(kMaxLds)) {
+    while (t.lds > cap && rows > 1) t = grid_tiling(pl, rows /= 2);
+    if (t.lds > cap) {
       b->separate.push_back(gi);
       continue;
     }
@@ -304,14 +337,17 @@ int build_tiles(sg_batch* b, int rows_per_wave) {
     const DimDev& d1 = pl.dims[0];
     const DimDev& d2 = pl.dims[1];
     const int chunk = rows * kWaves;
+    // 3-D grids: one tile per kWaves samples j3 (a plane per wave)
     const int m3 = pl.nin == 3 ? pl.dims[2].m : 1;
-    for (int z = 0; z < m3; ++z)
+    const int nz = pl.nin == 3 ? (m3 + kWaves - 1) / kWaves : 1;
+    for (int z = 0; z < nz; ++z)
       for (int y = 0; y < t.nty; ++y)
         for (int x = 0; x < t.ntx; ++x) {
           tiles.push_back(BTile{gi, x, y, z});
           const long long lanes = std::min(kLanes, d1.m - x * kLanes);
           const long long nrows = std::min(chunk, d2.m - y * chunk);
-          cost.push_back(lanes * nrows * pl.nout + 64LL * t.w1 * t.w2);
+          const long long planes = pl.nin == 3 ? std::min(kWaves, m3 - z * kWaves) : 1;
+          cost.push_back(lanes * nrows * planes * pl.nout + 64LL * planes * t.w1 * t.w2);
         }
     b->hgrids[gi].rows_per_wave = rows;
   }
@@ -337,6 +373,7 @@ int build_tiles(sg_batch* b, int rows_per_wave) {
   b->n_tiles = static_cast<int>(sorted.size());
   b->lds = lds;
   b->rows_per_wave = rows_per_wave;
+  b->blocks_per_cu = blocks_per_cu;
   return SG_OK;
 }
 
@@ -428,7 +465,7 @@ int sg_batch_create(const sg_grid_desc* grids, int32_t n_grids, void* stream, sg
     if (e == hipSuccess) e = hipMalloc(&b->counter, sizeof(int));
     if (e != hipSuccess) return bail(hip_fail(e, "batch allocation"));
   }
-  const int rc = build_tiles(b, 16);
+  const int rc = build_tiles(b, 16, 3);
   if (rc != SG_OK) return bail(rc);
   b->P.assign(n_grids, nullptr);
   b->W.assign(n_grids, nullptr);
@@ -538,10 +575,9 @@ int sg_batch_autotune(sg_batch* b, int32_t reps, void* stream) {
   float best = 1e30f;
   int best_rows = b->rows_per_wave, best_bpc = b->blocks_per_cu;
   int rc = SG_OK;
-  for (int rows : {8, 16, 32}) {
-    if ((rc = build_tiles(b, rows)) != SG_OK) continue;
-    for (int bpc : {2, 4, 8}) {
-      b->blocks_per_cu = bpc;
+  for (int bpc : {2, 3, 4, 6}) {
+    for (int rows : {8, 16, 32}) {
+      if ((rc = build_tiles(b, rows, bpc)) != SG_OK) continue;
       hipError_t e = launch_batch(b, s);  // warm-up
       if (e == hipSuccess) e = hipEventRecord(t0, s);
       for (int r = 0; r < reps && e == hipSuccess; ++r) e = launch_batch(b, s);
@@ -563,8 +599,7 @@ int sg_batch_autotune(sg_batch* b, int32_t reps, void* stream) {
   }
   (void)hipEventDestroy(t0);
   (void)hipEventDestroy(t1);
-  b->blocks_per_cu = best_bpc;
-  const int rc2 = build_tiles(b, best_rows);
+  const int rc2 = build_tiles(b, best_rows, best_bpc);
   if (prev != b->device) (void)hipSetDevice(prev);
   if (rc != SG_OK && rc != SG_ERR_UNSUPPORTED) return rc;
   return rc2;
@@ -578,6 +613,26 @@ int sg_batch_info(const sg_batch* b, int64_t* n_tiles, int64_t* n_samples, int32
   if (n_persistent) *n_persistent = static_cast<int32_t>(b->persistent.size());
   if (rows_per_wave) *rows_per_wave = b->rows_per_wave;
   if (blocks_per_cu) *blocks_per_cu = b->blocks_per_cu;
+  return SG_OK;
+}
+
+int sg_batch_describe(const sg_batch* b, char* text, int32_t capacity) {
+  if (!b || !text || capacity < 1) return fail(SG_ERR_INVALID, "NULL argument");
+  int resident = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          &resident, reinterpret_cast<const void*>(batch_persistent), kThreads, b->lds) !=
+      hipSuccess)
+    resident = -1;
+  std::ostringstream os;
+  os << "grids=" << b->plans.size() << " persistent=" << b->persistent.size()
+     << " separate=" << b->separate.size() << " tiles=" << b->n_tiles
+     << " samples=" << b->n_samples << " rows_per_wave=" << b->rows_per_wave
+     << " blocks_per_cu=" << b->blocks_per_cu << " resident_per_cu=" << resident
+     << " lds_bytes=" << b->lds << " cus=" << b->n_cu;
+  const std::string s = os.str();
+  const size_t n = std::min(s.size(), static_cast<size_t>(capacity - 1));
+  std::memcpy(text, s.data(), n);
+  text[n] = '\0';
   return SG_OK;
 }
 

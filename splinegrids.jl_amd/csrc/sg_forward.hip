@@ -211,7 +211,7 @@ template <typename T, bool NURBS>
 __global__ __launch_bounds__(256) void fwd_direct(DirectArgs<T> a) {
   const long long q = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
   if (q >= a.nsamp) return;
-  const T* tb[SG_MAX_NIN];
+  Global<const T>* tb[SG_MAX_NIN];
   int I[SG_MAX_NIN];
   long long base = 0, r = q;
 #pragma unroll
@@ -219,8 +219,8 @@ __global__ __launch_bounds__(256) void fwd_direct(DirectArgs<T> a) {
     if (d < a.nin) {
       const int j = static_cast<int>(r % a.m[d]);
       r /= a.m[d];
-      base += static_cast<long long>(a.span[d][j] - a.p[d]) * a.cps[d];
-      tb[d] = a.tab[d] + static_cast<size_t>(j) * (a.p[d] + 1);
+      base += static_cast<long long>(global_ptr(a.span[d])[j] - a.p[d]) * a.cps[d];
+      tb[d] = global_ptr(a.tab[d]) + static_cast<size_t>(j) * (a.p[d] + 1);
     }
   }
   for (int c0 = 0; c0 < a.nout; c0 += 4) {

@@ -42,6 +42,16 @@ __device__ inline void stage_to_lds(int n, Load load, Store store) {
   }
 }
 
+// A pointer into global memory, typed so.  Pointers read from memory (not
+// kernel arguments) are otherwise generic, and generic (FLAT) accesses count
+// on the LDS counter too, so every LDS wait would also wait for them.
+template <typename T>
+using Global = __attribute__((address_space(1))) T;
+template <typename T>
+__device__ inline Global<T>* global_ptr(T* p) {
+  return (Global<T>*)p;
+}
+
 // A value that every lane of the wave holds (read from one LDS address),
 // moved to a scalar register: branches on it stay scalar branches and FMAs
 // take it as a scalar operand.

@@ -66,7 +66,7 @@ def test_batch_autotune_keeps_results(gpu):
     torch.cuda.synchronize()
     ref = [g.eval.clone() for g in grids]
     info = batch.autotune(reps=2)
-    assert info["rows_per_wave"] in (8, 16, 32) and info["blocks_per_cu"] in (2, 4, 8)
+    assert info["rows_per_wave"] in (8, 16, 32) and info["blocks_per_cu"] in (2, 3, 4, 6)
     for g in grids:
         g.eval.zero_()
     batch.evaluate_()
