@@ -49,38 +49,162 @@ __global__ void batch_reset(int* counter) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *counter = 0;
 }
 
+// Everything a channel-block pass of a tile needs, set up once per tile.
+struct TileCtx {
+  Global<const int32_t>* span3;
+  Global<const float>* tab3;  // null for 2-D grids
+  Global<const float>* P;
+  Global<const float>* W;
+  Global<float>* out;
+  float* smem;
+  const float* RT;  // [chunk][P2+1] B2 rows of the chunk
+  const int* RS;    // [chunk] their spans
+  int n1, p3, pm1, nch, cs, W1, lo1, lo2, r0, rb, re, j3_0, j3, planes, q1b, m1, m2;
+  long long ncp, nsamp, plane_cp;
+  bool active;
+  int j1;
+  float b1[kMaxStreamDegree + 1];
+};
+
+// One channel block [c0, c0 + NC) of a tile: stage 1 (band windows of every
+// plane in LDS), then each wave's rows.  NC and NURBS are compile-time, so the
+// row loop does exactly NC (+ weight) window sums and stores.
+template <int P2, int NC, bool NURBS>
+__device__ __attribute__((always_inline)) void tile_pass(const TileCtx& x, int c0) {
+  constexpr int PM = kMaxStreamDegree;
+  constexpr int NCH = NC + (NURBS ? 1 : 0);
+  const int cs = x.cs;
+  // ---- stage 1: band windows, contracted along dimension 3 ----
+  for (int pidx = threadIdx.x; pidx < x.planes * cs; pidx += kThreads) {
+    const int pz = pidx / cs, idx = pidx - pz * cs;
+    float* Gp = x.smem + static_cast<size_t>(pz) * x.nch * cs;
+    const int q2 = idx / x.W1, q1 = idx - q2 * x.W1;
+    const long long cpi = static_cast<long long>(x.lo1 + q1) +
+                          static_cast<long long>(x.n1) * (x.lo2 + q2);
+    float acc[NCH];
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) acc[c] = 0.f;
+    if (x.tab3 == nullptr) {  // 2-D grid
+      const float w = NURBS ? x.W[cpi] : 1.f;
+#pragma unroll
+      for (int c = 0; c < NC; ++c) acc[c] = w * x.P[cpi + x.ncp * (c0 + c)];
+      if constexpr (NURBS) acc[NC] = w;
+    } else {
+      const int jz = x.j3_0 + pz;
+      const int s3 = x.span3[jz];
+      Global<const float>* b3 = x.tab3 + static_cast<size_t>(jz) * (x.p3 + 1);
+      for (int d = 0; d <= x.p3; ++d) {
+        const long long off = cpi + x.plane_cp * (s3 - x.p3 + d);
+        const float bw = NURBS ? b3[d] * x.W[off] : b3[d];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) acc[c] += bw * x.P[off + x.ncp * (c0 + c)];
+        if constexpr (NURBS) acc[NC] += bw;
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) Gp[c * cs + idx] = acc[c];
+  }
+  __syncthreads();
+
+  // ---- stage 2: rows of this wave, sliding window over k2 ----
+  if (x.rb < x.re && x.active) {
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kLanes);
+    // 3-D tiles: wave w reads plane w; 2-D tiles: all waves read plane 0
+    const float* G = x.smem + (x.tab3 != nullptr ? static_cast<size_t>(wave) * x.nch * cs : 0);
+    float V[P2 + 1][NCH];
+    auto column = [&](int k2, float (&v)[NCH]) {
+      const float* gr = G + (k2 - x.lo2) * x.W1 + x.q1b;
+#pragma unroll
+      for (int ch = 0; ch < NCH; ++ch) {
+        float s = 0.f;
+#pragma unroll
+        for (int q = 0; q <= PM; ++q) s += x.b1[q] * gr[ch * cs + min(q, x.pm1)];
+        v[ch] = s;
+      }
+    };
+    int cur = wave_uniform(x.RS[x.rb - x.r0]);
+#pragma unroll
+    for (int q = 0; q <= P2; ++q) column(cur - P2 + q, V[q]);
+    const long long obase = static_cast<long long>(x.j1) +
+                            static_cast<long long>(x.m1) * x.m2 * x.j3 + x.nsamp * c0;
+    for (int j2 = x.rb; j2 < x.re; ++j2) {
+      const int s2 = wave_uniform(x.RS[j2 - x.r0]);
+      if (s2 != cur) {
+        if (s2 - cur > P2) {
+          cur = s2;
+#pragma unroll
+          for (int q = 0; q <= P2; ++q) column(cur - P2 + q, V[q]);
+        } else {
+          while (cur < s2) {
+#pragma unroll
+            for (int q = 0; q < P2; ++q)
+#pragma unroll
+              for (int ch = 0; ch < NCH; ++ch) V[q][ch] = V[q + 1][ch];
+            ++cur;
+            column(cur, V[P2]);
+          }
+        }
+      }
+      const float* b2p = x.RT + static_cast<size_t>(j2 - x.r0) * (P2 + 1);
+      float bb[P2 + 1];
+#pragma unroll
+      for (int q = 0; q <= P2; ++q) bb[q] = wave_uniform(b2p[q]);
+      float num[NCH];
+#pragma unroll
+      for (int ch = 0; ch < NCH; ++ch) {
+        float s = 0.f;
+#pragma unroll
+        for (int q = 0; q <= P2; ++q) s += bb[q] * V[q][ch];
+        num[ch] = s;
+      }
+      const long long o = obase + static_cast<long long>(x.m1) * j2;
+#pragma unroll
+      for (int c = 0; c < NC; ++c) x.out[o + x.nsamp * c] = NURBS ? num[c] / num[NC] : num[c];
+    }
+  }
+  __syncthreads();  // G is rewritten by the next channel block / tile
+}
+
+template <int P2, bool NURBS>
+__device__ __attribute__((always_inline)) void tile_pass_nc(const TileCtx& x, int c0, int nc) {
+  switch (nc) {
+    case 1: tile_pass<P2, 1, NURBS>(x, c0); break;
+    case 2: tile_pass<P2, 2, NURBS>(x, c0); break;
+    case 3: tile_pass<P2, 3, NURBS>(x, c0); break;
+    default: tile_pass<P2, 4, NURBS>(x, c0); break;
+  }
+}
+
 template <int P2>
 __device__ void batch_tile(const BGrid& g, const BTile& t, float* smem, int lane, int wave) {
   constexpr int PM = kMaxStreamDegree;
   // the grid's arrays as global pointers (global loads and stores)
-  const auto span1 = global_ptr(g.span1), span2 = global_ptr(g.span2), span3 = global_ptr(g.span3);
-  const auto tab1 = global_ptr(g.tab1), tab2 = global_ptr(g.tab2), tab3 = global_ptr(g.tab3);
-  const auto Pg = global_ptr(g.P), Wg = global_ptr(g.W);
-  const auto outg = global_ptr(g.out);
+  const auto span1 = global_ptr(g.span1), span2 = global_ptr(g.span2);
+  const auto tab1 = global_ptr(g.tab1), tab2 = global_ptr(g.tab2);
   const int chunk_cap = g.rows_per_wave * kWaves;
   const int j1_0 = t.bx * kLanes;
   const int j1_last = min(j1_0 + kLanes - 1, g.m1 - 1);
   const int r0 = t.by * chunk_cap, r1 = min(r0 + chunk_cap, g.m2);
   const int p1 = g.p1;
-  const int lo1 = span1[j1_0] - p1;
-  const int W1 = span1[j1_last] - lo1 + 1;
-  const int lo2 = span2[r0] - P2;
-  const int W2 = span2[r1 - 1] - lo2 + 1;
-  const int cs = W1 * W2;
+  TileCtx x;
+  x.lo1 = span1[j1_0] - p1;
+  x.W1 = span1[j1_last] - x.lo1 + 1;
+  x.lo2 = span2[r0] - P2;
+  const int W2 = span2[r1 - 1] - x.lo2 + 1;
+  x.cs = x.W1 * W2;
   const bool nurbs = g.nurbs != 0;
-  const int nch = min(kBC, g.nout) + (nurbs ? 1 : 0);
+  x.nch = min(kBC, g.nout) + (nurbs ? 1 : 0);
   // 3-D grids: the tile holds kWaves consecutive samples j3 (from
   // kWaves * t.bz), one per wave, and every wave walks all rows of the chunk;
   // 2-D grids: one plane, the waves split the chunk's rows
   const bool zpack = g.nin == 3;
-  const int j3_0 = zpack ? t.bz * kWaves : 0;
-  const int planes = zpack ? min(kWaves, g.m3 - j3_0) : 1;
+  x.j3_0 = zpack ? t.bz * kWaves : 0;
+  x.planes = zpack ? min(kWaves, g.m3 - x.j3_0) : 1;
   // [planes][nch][cs] window slabs (weight last), then the chunk's B2 rows
   // and spans (space for kWaves planes is reserved for 3-D grids, as the
   // host's LDS size assumes)
-  float* RT = smem + static_cast<size_t>(zpack ? kWaves : 1) * nch * cs;  // [chunk][P2+1]
+  float* RT = smem + static_cast<size_t>(zpack ? kWaves : 1) * x.nch * x.cs;
   int* RS = reinterpret_cast<int*>(RT + static_cast<size_t>(chunk_cap) * (P2 + 1));
-
   {
     const size_t rbase = static_cast<size_t>(r0) * (P2 + 1);
     stage_to_lds<1, 1>(r1 - r0, [&](int, int i) { return span2[r0 + i]; },
@@ -88,124 +212,42 @@ __device__ void batch_tile(const BGrid& g, const BTile& t, float* smem, int lane
     stage_to_lds<1, 4>((r1 - r0) * (P2 + 1), [&](int, int i) { return tab2[rbase + i]; },
                        [&](int, int i, float v) { RT[i] = v; });
   }
-
-  const int rb = zpack ? (wave < planes ? r0 : r1) : r0 + wave * g.rows_per_wave;
-  const int re = zpack ? r1 : min(rb + g.rows_per_wave, r1);
-  const int j3 = j3_0 + (zpack ? wave : 0);
-  const float* G = smem + (zpack ? static_cast<size_t>(wave) * nch * cs : 0);
-  const bool active = (j1_0 + lane) < g.m1;
-  const int j1 = active ? j1_0 + lane : g.m1 - 1;
-  const int q1b = span1[j1] - p1 - lo1;
+  x.smem = smem;
+  x.RT = RT;
+  x.RS = RS;
+  x.span3 = zpack ? global_ptr(g.span3) : nullptr;
+  x.tab3 = zpack ? global_ptr(g.tab3) : nullptr;
+  x.P = global_ptr(g.P);
+  x.W = global_ptr(g.W);
+  x.out = global_ptr(g.out);
+  x.n1 = g.n1;
+  x.p3 = g.p3;
+  x.pm1 = p1;
+  x.r0 = r0;
+  x.rb = zpack ? (wave < x.planes ? r0 : r1) : r0 + wave * g.rows_per_wave;
+  x.re = zpack ? r1 : min(x.rb + g.rows_per_wave, r1);
+  x.j3 = x.j3_0 + (zpack ? wave : 0);
+  x.m1 = g.m1;
+  x.m2 = g.m2;
+  x.ncp = g.ncp;
+  x.nsamp = g.nsamp;
+  x.plane_cp = static_cast<long long>(g.n1) * g.n2;
+  x.active = (j1_0 + lane) < g.m1;
+  x.j1 = x.active ? j1_0 + lane : g.m1 - 1;
+  x.q1b = span1[x.j1] - p1 - x.lo1;
   // b1 is zero past p1, so the window sums take all PM+1 terms without a
   // branch (the clamped index re-reads the last element of the band)
-  float b1[PM + 1];
 #pragma unroll
   for (int q = 0; q <= PM; ++q) {
-    const float v = tab1[static_cast<size_t>(j1) * (p1 + 1) + min(q, p1)];
-    b1[q] = q <= p1 ? v : 0.f;
+    const float v = tab1[static_cast<size_t>(x.j1) * (p1 + 1) + min(q, p1)];
+    x.b1[q] = q <= p1 ? v : 0.f;
   }
-
   for (int c0 = 0; c0 < g.nout; c0 += kBC) {
     const int nc = min(kBC, g.nout - c0);
-    // ---- stage 1: band windows, contracted along dimension 3 ----
-    for (int pidx = threadIdx.x; pidx < planes * cs; pidx += kThreads) {
-      const int pz = pidx / cs, idx = pidx - pz * cs;
-      const int jz = j3_0 + pz;
-      float* Gp = smem + static_cast<size_t>(pz) * nch * cs;
-      const int q2 = idx / W1, q1 = idx - q2 * W1;
-      const long long cpi =
-          static_cast<long long>(lo1 + q1) + static_cast<long long>(g.n1) * (lo2 + q2);
-      float acc[kBC] = {0.f, 0.f, 0.f, 0.f};
-      float accw = 0.f;
-      if (g.nin == 2) {
-        float w = 1.f;
-        if (nurbs) w = Wg[cpi];
-        accw = w;
-#pragma unroll
-        for (int c = 0; c < kBC; ++c)
-          if (c < nc) acc[c] = w * Pg[cpi + g.ncp * (c0 + c)];
-      } else {
-        const int s3 = span3[jz];
-        const auto b3 = tab3 + static_cast<size_t>(jz) * (g.p3 + 1);
-        const long long plane = static_cast<long long>(g.n1) * g.n2;
-        for (int d = 0; d <= g.p3; ++d) {
-          const long long off = cpi + plane * (s3 - g.p3 + d);
-          float bw = b3[d];
-          if (nurbs) bw *= Wg[off];
-          accw += bw;
-#pragma unroll
-          for (int c = 0; c < kBC; ++c)
-            if (c < nc) acc[c] += bw * Pg[off + g.ncp * (c0 + c)];
-        }
-      }
-#pragma unroll
-      for (int c = 0; c < kBC; ++c)
-        if (c < nc) Gp[c * cs + idx] = acc[c];
-      if (nurbs) Gp[nc * cs + idx] = accw;
-    }
-    __syncthreads();
-
-    // ---- stage 2: rows of this wave, sliding window over k2 ----
-    if (rb < re) {
-      float V[P2 + 1][kBC + 1];
-      auto column = [&](int k2, float (&v)[kBC + 1]) {
-        const float* gr = G + (k2 - lo2) * W1 + q1b;
-#pragma unroll
-        for (int ch = 0; ch <= kBC; ++ch) {
-          // register slot kBC holds the weight, whose LDS slab is nc
-          const int slab = ch == kBC ? nc : ch;
-          float s = 0.f;
-          if (ch < nc || (ch == kBC && nurbs)) {
-#pragma unroll
-            for (int q = 0; q <= PM; ++q) s += b1[q] * gr[slab * cs + min(q, p1)];
-          }
-          v[ch] = s;
-        }
-      };
-      int cur = wave_uniform(RS[rb - r0]);
-#pragma unroll
-      for (int q = 0; q <= P2; ++q) column(cur - P2 + q, V[q]);
-      const long long obase = static_cast<long long>(j1) +
-                              static_cast<long long>(g.m1) * g.m2 * j3 + g.nsamp * c0;
-      for (int j2 = rb; j2 < re; ++j2) {
-        const int s2 = wave_uniform(RS[j2 - r0]);
-        if (s2 != cur) {
-          if (s2 - cur > P2) {
-            cur = s2;
-#pragma unroll
-            for (int q = 0; q <= P2; ++q) column(cur - P2 + q, V[q]);
-          } else {
-            while (cur < s2) {
-#pragma unroll
-              for (int q = 0; q < P2; ++q)
-#pragma unroll
-                for (int ch = 0; ch <= kBC; ++ch) V[q][ch] = V[q + 1][ch];
-              ++cur;
-              column(cur, V[P2]);
-            }
-          }
-        }
-        const float* b2p = RT + static_cast<size_t>(j2 - r0) * (P2 + 1);
-        float bb[P2 + 1];
-#pragma unroll
-        for (int q = 0; q <= P2; ++q) bb[q] = wave_uniform(b2p[q]);
-        float num[kBC + 1];
-#pragma unroll
-        for (int ch = 0; ch <= kBC; ++ch) {
-          float s = 0.f;
-#pragma unroll
-          for (int q = 0; q <= P2; ++q) s += bb[q] * V[q][ch];
-          num[ch] = s;
-        }
-        if (active) {
-          const long long o = obase + static_cast<long long>(g.m1) * j2;
-#pragma unroll
-          for (int c = 0; c < kBC; ++c)
-            if (c < nc) outg[o + g.nsamp * c] = nurbs ? num[c] / num[kBC] : num[c];
-        }
-      }
-    }
-    __syncthreads();  // G is rewritten by the next channel block / tile
+    if (nurbs)
+      tile_pass_nc<P2, true>(x, c0, nc);
+    else
+      tile_pass_nc<P2, false>(x, c0, nc);
   }
 }
 
