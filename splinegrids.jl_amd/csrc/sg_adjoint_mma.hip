@@ -41,7 +41,7 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 
 constexpr int kTile1 = 64;     // samples j1 per task
 constexpr int kMmaWin = 16;    // control points per MFMA tile edge
-constexpr int kBufs = 4;  // ebar units per unrolled group; kBufs - 1 are in flight
+constexpr int kBufs = 8;  // ebar units per unrolled group; kBufs - 1 are in flight
 
 struct AdjMmaArgs {
   const float* E;
@@ -130,12 +130,11 @@ __global__ __launch_bounds__(kThreads) void adj_mma(AdjMmaArgs a) {
       }
   }
 
-  // Units u = (c * (j3b - j3a) + (j3 - j3a)) * NB + b in order, channels
-  // outermost: unit u is row r0 + 16 b + i (lane (g, i)) of sample j3 and
-  // channel c0 + c; load q holds samples j1_0 + 16 q + 4 g .. + 3.  A unit
-  // index past the last unit reads nothing (every offset out of range).
-  const int per_c = (j3b - j3a) * NB;
-  const int U = a.nc * per_c;
+  // Units in order: channel c outermost, then sample j3, then row block b;
+  // unit (c, j3, b) is row r0 + 16 b + i (lane (g, i)) of sample j3 and
+  // channel c0 + c, and load q holds samples j1_0 + 16 q + 4 g .. + 3.
+  const int J = j3b - j3a;
+  const int U = a.nc * J * NB;
   // byte offset of (row block b, load q) within a plane, or kOff
   unsigned voff[NB];
 #pragma unroll
@@ -144,13 +143,15 @@ __global__ __launch_bounds__(kThreads) void adj_mma(AdjMmaArgs a) {
     voff[b] = j2 < r1 ? static_cast<unsigned>((j2 * a.m1 + j1_0 + 4 * g) * 4) : kOff;
   }
   const unsigned plane_bytes = static_cast<unsigned>(a.m12 * 4);
-  auto load = [&](f4 (&e)[4], int u, int b) {
-    const int c = u / per_c;
-    const int jl = (u - c * per_c) / NB;
-    const unsigned ok = u < U ? ~0u : 0u;
-    const float* base = a.E + static_cast<long long>(j3a + jl) * a.m12 + a.nsamp * (a.c0 + c);
+  // Load cursor: the (channel, sample) plane of the next unit to load, kept
+  // as running scalar state (no division per load).  Units past the last
+  // read nothing: their descriptor has no records.
+  int lc = 0, ljl = 0;
+  const float* lplane = a.E + static_cast<long long>(j3a) * a.m12 + a.nsamp * a.c0;
+  auto load = [&](f4 (&e)[4], int b) {
     const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
-        const_cast<float*>(base), 0, static_cast<int>(plane_bytes & ok), kRsrcWord3);
+        const_cast<float*>(lplane), 0, static_cast<int>(lc < a.nc ? plane_bytes : 0u),
+        kRsrcWord3);
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int j1 = j1_0 + 16 * q + 4 * g;
@@ -166,6 +167,14 @@ __global__ __launch_bounds__(kThreads) void adj_mma(AdjMmaArgs a) {
         }
       }
     }
+    if (b == NB - 1) {  // the plane's last row block: advance the cursor
+      lplane += a.m12;
+      if (++ljl == J) {
+        ljl = 0;
+        ++lc;
+        lplane = a.E + static_cast<long long>(j3a) * a.m12 + a.nsamp * (a.c0 + lc);
+      }
+    }
   };
 
   // Units are taken in unrolled groups of kBufs with one register buffer per
@@ -176,7 +185,7 @@ __global__ __launch_bounds__(kThreads) void adj_mma(AdjMmaArgs a) {
   // position k, k % NB, a constant.
   f4 buf[kBufs][4];
 #pragma unroll
-  for (int k = 0; k < kBufs - 1; ++k) load(buf[k], k, k % NB);
+  for (int k = 0; k < kBufs - 1; ++k) load(buf[k], k % NB);
 
   f4 R[P3 + 1];  // ring of the current channel: slot q holds k3 = cur3 - P3 + q
 #pragma unroll
@@ -201,12 +210,13 @@ __global__ __launch_bounds__(kThreads) void adj_mma(AdjMmaArgs a) {
       if (4 * g + r < W2) dst[(4 * g + r) * kMmaWin + i] = slab[r];
   };
 
-  const int J = j3b - j3a;
+  // compute cursor: channel and sample of the unit being multiplied
+  int cc = 0, cjl = 0;
   for (int u0 = 0; u0 < U; u0 += kBufs) {
 #pragma unroll
     for (int k = 0; k < kBufs; ++k) {
       const int u = u0 + k;
-      load(buf[(k + kBufs - 1) % kBufs], u + kBufs - 1, (k + kBufs - 1) % NB);
+      load(buf[(k + kBufs - 1) % kBufs], (k + kBufs - 1) % NB);
       if (u < U) {
         const int b = k % NB;
         f4 d1 = {0.f, 0.f, 0.f, 0.f};
@@ -218,9 +228,12 @@ __global__ __launch_bounds__(kThreads) void adj_mma(AdjMmaArgs a) {
           acc = __builtin_amdgcn_mfma_f32_16x16x4f32(b2[b][r], d1[r], acc, 0, 0, 0);
         if (b == NB - 1) {
           // T2 of sample j3 is complete: weight it into the ring
-          const int c = u / per_c;
-          const int jl = (u - c * per_c) / NB;
+          const int c = cc, jl = cjl;
           const int j3 = j3a + jl;
+          if (++cjl == J) {
+            cjl = 0;
+            ++cc;
+          }
           float b3[P3 + 1];
 #pragma unroll
           for (int q = 0; q <= P3; ++q) b3[q] = b3n[q];
