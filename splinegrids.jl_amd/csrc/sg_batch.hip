@@ -158,8 +158,9 @@ __device__ __attribute__((always_inline)) void tile_pass(const TileCtx& x, int c
         num[ch] = s;
       }
       const long long o = obase + static_cast<long long>(x.m1) * j2;
+      const float inv = NURBS ? nurbs_recip(num[NC]) : 1.f;
 #pragma unroll
-      for (int c = 0; c < NC; ++c) x.out[o + x.nsamp * c] = NURBS ? num[c] / num[NC] : num[c];
+      for (int c = 0; c < NC; ++c) x.out[o + x.nsamp * c] = NURBS ? num[c] * inv : num[c];
     }
   }
   __syncthreads();  // G is rewritten by the next channel block / tile

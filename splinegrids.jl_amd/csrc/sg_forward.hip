@@ -17,7 +17,7 @@
 // Lanes run along dimension 1 (the fastest axis of eval), so each store
 // instruction of a wave writes 64 contiguous elements.  Row-uniform data
 // (spans and B2 of row j2) is read with scalar loads.  NURBS carry the
-// weight as one more channel and divide once at the store.
+// weight as one more channel; each sample takes one reciprocal of it.
 //
 // A direct kernel (one thread per sample, all (p+1)^Nin terms, terms in the
 // reference's column-major order) covers every case the streaming kernel
@@ -178,9 +178,10 @@ __global__ __launch_bounds__(kThreads) void fwd_stream(
         for (int q = 0; q <= P2; ++q) s += bb[f][q] * V[q][f][ch];
         num[ch] = s;
       }
+      const T inv = NURBS ? nurbs_recip(num[NCH - 1]) : T(1);
 #pragma unroll
       for (int c = 0; c < NC; ++c) {
-        const T v = NURBS ? num[c] / num[NCH - 1] : num[c];
+        const T v = NURBS ? num[c] * inv : num[c];
         T* dst = outs[f] + orow + a.nsamp * c;
         if (a.nt)
           __builtin_nontemporal_store(v, dst);

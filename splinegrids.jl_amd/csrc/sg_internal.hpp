@@ -52,6 +52,12 @@ __device__ inline Global<T>* global_ptr(T* p) {
   return (Global<T>*)p;
 }
 
+// 1/x for the NURBS quotient: FP32 takes the hardware reciprocal (at most
+// 1 ulp from the correctly rounded value; 1/0 = inf, so 0/0 stays NaN), FP64
+// one IEEE division.  Each sample then multiplies its channels by it.
+__device__ inline float nurbs_recip(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ inline double nurbs_recip(double x) { return 1.0 / x; }
+
 // A value that every lane of the wave holds (read from one LDS address),
 // moved to a scalar register: branches on it stay scalar branches and FMAs
 // take it as a scalar operand.
