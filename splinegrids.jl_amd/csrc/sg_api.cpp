@@ -447,13 +447,18 @@ int build_plan(const sg_grid_desc* desc, hipStream_t s, bool want_adjoint, Plan*
     }
     pl->fwd.xcd_map = xcd;
     // MFMA tiers: 64-sample tiles, each wave 16 samples of dimension 1
+    // (LDS: the widest channel block, or four fields of one channel)
+    const int mnch = std::max(std::min(pl->nout, 4) + (pl->nurbs ? 1 : 0), 1);
+    auto mma_worst = [&]() {
+      return std::max(mma_lds(*pl, mnch, 1), mma_lds(*pl, 1, SG_MAX_FIELDS));
+    };
     int mrows = 32;
     compute_tiling(*pl, pl->mma, mrows, 1);
-    while (forward_lds_worst(*pl, pl->mma) > static_cast<size_t>(kMaxLds) && mrows > 4) {
+    while (mma_worst() > static_cast<size_t>(kMaxLds) && mrows > 4) {
       mrows /= 2;
       compute_tiling(*pl, pl->mma, mrows, 1);
     }
-    if (forward_lds_worst(*pl, pl->mma) > static_cast<size_t>(kMaxLds)) pl->mma = Tiling();
+    if (mma_worst() > static_cast<size_t>(kMaxLds)) pl->mma = Tiling();
     pl->mma.xcd_map = xcd;
     ok = forward_lds_worst(*pl, pl->fwd) <= static_cast<size_t>(kMaxLds);
   }

@@ -188,6 +188,8 @@ hipError_t launch_forward(const Plan& plan, int nf, const int32_t* orders,
 
 // ---- sg_mfma.hip ----
 bool mma_supported(const Plan& plan);
+// LDS bytes of an MFMA forward launch: nch window channels, nf fields.
+size_t mma_lds(const Plan& plan, int nch, int nf);
 hipError_t launch_forward_mma(const Plan& plan, int tier, int nf, const int32_t* orders,
                               const float* P, const float* W, float* const* outs,
                               hipStream_t s);

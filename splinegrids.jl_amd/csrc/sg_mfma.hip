@@ -17,6 +17,8 @@
 // where B2d is the banded B2 made dense on the fly.  V^T is kept while the
 // rows' bands stay inside [kb, kb+16).  MFMA operand layout (gfx9 16x16x16):
 // lane l = 16 g + i holds A[i][4g..4g+3], B[4g..4g+3][i], D[4g..4g+3][i].
+#include <vector>
+
 #include "sg_tiles.hpp"
 
 namespace sg {
@@ -29,17 +31,18 @@ typedef float float4_t __attribute__((ext_vector_type(4)));
 
 struct MmaArgs {
   BandSrc<float> src;
-  float* out;
-  const float* tab1;  // [m1][p1+1]
-  const float* tab2;  // [m2][p2+1]
+  float* out[SG_MAX_FIELDS];
+  const float* tab1[SG_MAX_FIELDS];  // [m1][p1+1] at each field's order
+  const float* tab2[SG_MAX_FIELDS];  // [m2][p2+1]
   const int32_t* span1;
   const int32_t* span2;
   int m1, m2, p1, p2;
   int rows_per_chunk;
   int ntx, nty, xcd_map;
-  int gmax;            // LDS floats reserved per channel of G
-  float s1, s2;        // 2^-e of dimension 1 / 2 tables at the requested orders
-  int e12;             // e1 + e2
+  int gmax;                      // LDS floats reserved per channel of G
+  float s1[SG_MAX_FIELDS];       // 2^-e of the dimension 1 / 2 tables of each field
+  float s2[SG_MAX_FIELDS];
+  int e12[SG_MAX_FIELDS];        // e1 + e2 of each field
   long long nsamp;
 };
 
@@ -77,7 +80,7 @@ struct Operand<SG_TIER_BF16> {
   }
 };
 
-template <int TIER, int NIN, int NC, bool NURBS>
+template <int TIER, int NIN, int NF, int NC, bool NURBS>
 __global__ __launch_bounds__(kThreads) void mma_stream(MmaArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr int NCH = NC + (NURBS ? 1 : 0);
@@ -97,18 +100,19 @@ __global__ __launch_bounds__(kThreads) void mma_stream(MmaArgs a) {
   const int hi2 = a.span2[r1 - 1];
   const int W2 = hi2 - lo2 + 1;
   const int cs = W1 * W2;
-  // the chunk's B2 rows and spans in LDS (no vector-memory loads in the
-  // row loop, whose stores would otherwise be waited for)
-  float* RT = G + static_cast<size_t>(NCH) * a.gmax;
-  int* RS = reinterpret_cast<int*>(RT + static_cast<size_t>(a.rows_per_chunk) * (p2 + 1));
+  // the chunk's B2 rows of every field (scaled) and spans in LDS
+  const int chunk = a.rows_per_chunk;
+  float* RT = G + static_cast<size_t>(NCH) * a.gmax;                        // [NF][chunk][p2+1]
+  int* RS = reinterpret_cast<int*>(RT + static_cast<size_t>(NF) * chunk * (p2 + 1));
 
   stage1_band_window<float, NIN, NC, NURBS>(G, a.src, j3, lo1, W1, lo2, W2);
   {
     const size_t rbase = static_cast<size_t>(r0) * (p2 + 1);
+    const size_t fstride = static_cast<size_t>(chunk) * (p2 + 1);
     stage_to_lds<1, 1>(r1 - r0, [&](int, int i) { return a.span2[r0 + i]; },
                        [&](int, int i, int v) { RS[i] = v; });
-    stage_to_lds<1, 4>((r1 - r0) * (p2 + 1), [&](int, int i) { return a.tab2[rbase + i]; },
-                       [&](int, int i, float v) { RT[i] = v * a.s2; });
+    stage_to_lds<NF, 4>((r1 - r0) * (p2 + 1), [&](int f, int i) { return a.tab2[f][rbase + i]; },
+                        [&](int f, int i, float v) { RT[f * fstride + i] = v * a.s2[f]; });
   }
   __syncthreads();
 
@@ -143,13 +147,18 @@ __global__ __launch_bounds__(kThreads) void mma_stream(MmaArgs a) {
   const bool active = j1q < a.m1;
   const int j1c = active ? j1q : a.m1 - 1;
   const int q1b = a.span1[j1c] - p1 - lo1;
-  float b1[PM + 1];
+  // b1 is zero past p1 (scaled by the field's 2^-e1)
+  float b1[NF][PM + 1];
 #pragma unroll
-  for (int t = 0; t <= PM; ++t)
-    b1[t] = (t <= p1) ? a.tab1[static_cast<size_t>(j1c) * (p1 + 1) + min(t, p1)] * a.s1 : 0.f;
+  for (int f = 0; f < NF; ++f)
+#pragma unroll
+    for (int t = 0; t <= PM; ++t) {
+      const float v = a.tab1[f][static_cast<size_t>(j1c) * (p1 + 1) + min(t, p1)] * a.s1[f];
+      b1[f][t] = t <= p1 ? v : 0.f;
+    }
 
-  Operand<TIER> vop[NCH];
-  int kb = -(1 << 30);  // k2 base of the cached V^T operand
+  Operand<TIER> vop[NF][NCH];
+  int kb = -(1 << 30);  // k2 base of the cached V^T operands
   auto build_v = [&](int base) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
@@ -158,11 +167,16 @@ __global__ __launch_bounds__(kThreads) void mma_stream(MmaArgs a) {
       const float* gr = G + (in ? (k2 - lo2) : 0) * W1 + q1b;
 #pragma unroll
       for (int ch = 0; ch < NCH; ++ch) {
-        float v = 0.f;
+        float gv[PM + 1];
 #pragma unroll
-        for (int t = 0; t <= PM; ++t)
-          if (t <= p1) v += b1[t] * gr[ch * cs + min(t, p1)];
-        vop[ch].set(j, in ? v * gs[ch] : 0.f);
+        for (int t = 0; t <= PM; ++t) gv[t] = gr[ch * cs + min(t, p1)];
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+          float v = 0.f;
+#pragma unroll
+          for (int t = 0; t <= PM; ++t) v += b1[f][t] * gv[t];
+          vop[f][ch].set(j, in ? v * gs[ch] : 0.f);
+        }
       }
     }
   };
@@ -171,54 +185,56 @@ __global__ __launch_bounds__(kThreads) void mma_stream(MmaArgs a) {
                           a.nsamp * a.src.c0;
   for (int jb = r0; jb < r1; jb += 16) {
     const int jl = min(jb + 15, r1 - 1);
-    const int klo = RS[jb - r0] - p2, khi = RS[jl - r0];
-    // this lane's row of the A operand
+    const int klo = wave_uniform(RS[jb - r0]) - p2, khi = wave_uniform(RS[jl - r0]);
+    // this lane's row of the A operands
     const bool rowok = jb + col < r1;
     const int j2a = rowok ? jb + col : r1 - 1;
     const int base2 = RS[j2a - r0] - p2;
-    float b2[PM + 1];
+    float4_t acc[NF][NCH];
 #pragma unroll
-    for (int t = 0; t <= PM; ++t)
-      b2[t] = (rowok && t <= p2) ? RT[(j2a - r0) * (p2 + 1) + min(t, p2)] : 0.f;
-    float4_t acc[NCH];
+    for (int f = 0; f < NF; ++f)
 #pragma unroll
-    for (int ch = 0; ch < NCH; ++ch) acc[ch] = float4_t{0.f, 0.f, 0.f, 0.f};
+      for (int ch = 0; ch < NCH; ++ch) acc[f][ch] = float4_t{0.f, 0.f, 0.f, 0.f};
     int kc = (klo >= kb && khi <= kb + 15) ? kb : klo;
     for (; kc <= khi; kc += 16) {
       if (kc != kb) {
         kb = kc;
         build_v(kb);
       }
-      Operand<TIER> aop;
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int off = kc + 4 * g + j - base2;
-        float v = 0.f;
+      for (int f = 0; f < NF; ++f) {
+        // A[col][4g + j] = B2 of row j2a at k2 = kc + 4g + j (0 off the band)
+        const float* rt = RT + (static_cast<size_t>(f) * chunk + (j2a - r0)) * (p2 + 1);
+        Operand<TIER> aop;
 #pragma unroll
-        for (int t = 0; t <= PM; ++t)
-          if (off == t) v = b2[t];
-        aop.set(j, v);
+        for (int j = 0; j < 4; ++j) {
+          const int off = kc + 4 * g + j - base2;
+          const float v = rt[min(max(off, 0), p2)];
+          aop.set(j, (rowok && off >= 0 && off <= p2) ? v : 0.f);
+        }
+#pragma unroll
+        for (int ch = 0; ch < NCH; ++ch) acc[f][ch] = Operand<TIER>::mma(aop, vop[f][ch], acc[f][ch]);
       }
-#pragma unroll
-      for (int ch = 0; ch < NCH; ++ch) acc[ch] = Operand<TIER>::mma(aop, vop[ch], acc[ch]);
     }
     if (active) {
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int j2 = jb + 4 * g + j;
-        if (j2 >= r1) continue;
-        const long long o = obase + static_cast<long long>(a.m1) * j2;
-        if constexpr (NURBS) {
-          const float den = ldexpf(acc[NC][j], eg[NC] + a.e12);
+      for (int f = 0; f < NF; ++f)
 #pragma unroll
-          for (int c = 0; c < NC; ++c)
-            a.out[o + a.nsamp * c] = ldexpf(acc[c][j], eg[c] + a.e12) / den;
-        } else {
+        for (int j = 0; j < 4; ++j) {
+          const int j2 = jb + 4 * g + j;
+          if (j2 >= r1) continue;
+          const long long o = obase + static_cast<long long>(a.m1) * j2;
+          if constexpr (NURBS) {
+            const float inv = nurbs_recip(ldexpf(acc[f][NC][j], eg[NC] + a.e12[f]));
 #pragma unroll
-          for (int c = 0; c < NC; ++c)
-            a.out[o + a.nsamp * c] = ldexpf(acc[c][j], eg[c] + a.e12);
+            for (int c = 0; c < NC; ++c)
+              a.out[f][o + a.nsamp * c] = ldexpf(acc[f][c][j], eg[c] + a.e12[f]) * inv;
+          } else {
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+              a.out[f][o + a.nsamp * c] = ldexpf(acc[f][c][j], eg[c] + a.e12[f]);
+          }
         }
-      }
     }
   }
 }
@@ -226,35 +242,51 @@ __global__ __launch_bounds__(kThreads) void mma_stream(MmaArgs a) {
 using MmaKernel = void (*)(MmaArgs);
 
 template <int TIER, int NIN>
-MmaKernel pick_mma_nc(int nc, bool nurbs) {
+MmaKernel pick_mma_nc(int nf, int nc, bool nurbs) {
   if (nurbs) {
     switch (nc) {
-      case 1: return mma_stream<TIER, NIN, 1, true>;
-      case 2: return mma_stream<TIER, NIN, 2, true>;
-      case 3: return mma_stream<TIER, NIN, 3, true>;
+      case 1: return mma_stream<TIER, NIN, 1, 1, true>;
+      case 2: return mma_stream<TIER, NIN, 1, 2, true>;
+      case 3: return mma_stream<TIER, NIN, 1, 3, true>;
     }
     return nullptr;
   }
-  switch (nc) {
-    case 1: return mma_stream<TIER, NIN, 1, false>;
-    case 2: return mma_stream<TIER, NIN, 2, false>;
-    case 3: return mma_stream<TIER, NIN, 3, false>;
-    case 4: return mma_stream<TIER, NIN, 4, false>;
+  if (nf == 1) {
+    switch (nc) {
+      case 1: return mma_stream<TIER, NIN, 1, 1, false>;
+      case 2: return mma_stream<TIER, NIN, 1, 2, false>;
+      case 3: return mma_stream<TIER, NIN, 1, 3, false>;
+      case 4: return mma_stream<TIER, NIN, 1, 4, false>;
+    }
+    return nullptr;
   }
+  if (nc != 1) return nullptr;
+  if (nf == 2) return mma_stream<TIER, NIN, 2, 1, false>;
+  if (nf == 3) return mma_stream<TIER, NIN, 3, 1, false>;
+  if (nf == 4) return mma_stream<TIER, NIN, 4, 1, false>;
   return nullptr;
 }
 
-MmaKernel pick_mma(int tier, int nin, int nc, bool nurbs) {
+MmaKernel pick_mma(int tier, int nin, int nf, int nc, bool nurbs) {
   if (tier == SG_TIER_SPLIT_F16)
-    return nin == 2 ? pick_mma_nc<SG_TIER_SPLIT_F16, 2>(nc, nurbs)
-                    : pick_mma_nc<SG_TIER_SPLIT_F16, 3>(nc, nurbs);
-  return nin == 2 ? pick_mma_nc<SG_TIER_BF16, 2>(nc, nurbs)
-                  : pick_mma_nc<SG_TIER_BF16, 3>(nc, nurbs);
+    return nin == 2 ? pick_mma_nc<SG_TIER_SPLIT_F16, 2>(nf, nc, nurbs)
+                    : pick_mma_nc<SG_TIER_SPLIT_F16, 3>(nf, nc, nurbs);
+  return nin == 2 ? pick_mma_nc<SG_TIER_BF16, 2>(nf, nc, nurbs)
+                  : pick_mma_nc<SG_TIER_BF16, 3>(nf, nc, nurbs);
 }
 
 }  // namespace
 
 bool mma_supported(const Plan& pl) { return pl.stream_ok && pl.mma.n_tiles1 > 0; }
+
+// LDS of mma_stream: G for nch channels, the chunk's B2 rows of nf fields and
+// its spans.
+size_t mma_lds(const Plan& pl, int nch, int nf) {
+  const Tiling& t = pl.mma;
+  return (static_cast<size_t>(nch) * t.w1max * t.w2max +
+          static_cast<size_t>(nf) * t.chunk_rows() * (pl.dims[1].p + 1)) * sizeof(float) +
+         static_cast<size_t>(t.chunk_rows()) * sizeof(int);
+}
 
 hipError_t launch_forward_mma(const Plan& pl, int tier, int nf, const int32_t* orders,
                               const float* P, const float* W, float* const* outs,
@@ -293,28 +325,42 @@ hipError_t launch_forward_mma(const Plan& pl, int tier, int nf, const int32_t* o
     gz = static_cast<unsigned>(pl.dims[2].m);
   }
   const dim3 grid(tl.n_tiles1 * tl.n_chunks2 * gz);
-  for (int f = 0; f < nf; ++f) {
-    const int o1 = orders[f * nin + 0], o2 = orders[f * nin + 1];
-    a.tab1 = d1.ktab32 + static_cast<size_t>(o1) * d1.m * (d1.p + 1);
-    a.tab2 = d2.ktab32 + static_cast<size_t>(o2) * d2.m * (d2.p + 1);
+  // Fields that share the dimension-3 order share stage 1: launch each group
+  // of up to 4 fields (B-splines; NURBS take one field per launch) and each
+  // channel block.
+  std::vector<int> done(nf, 0);
+  for (int f0 = 0; f0 < nf; ++f0) {
+    if (done[f0]) continue;
+    std::vector<int> group;
+    for (int f = f0; f < nf && static_cast<int>(group.size()) < SG_MAX_FIELDS; ++f) {
+      if (done[f]) continue;
+      if (nin == 3 && orders[f * nin + 2] != orders[f0 * nin + 2]) continue;
+      group.push_back(f);
+      done[f] = 1;
+      if (nurbs) break;
+    }
+    const int gnf = static_cast<int>(group.size());
+    for (int gi = 0; gi < gnf; ++gi) {
+      const int f = group[gi];
+      const int o1 = orders[f * nin + 0], o2 = orders[f * nin + 1];
+      a.tab1[gi] = d1.ktab32 + static_cast<size_t>(o1) * d1.m * (d1.p + 1);
+      a.tab2[gi] = d2.ktab32 + static_cast<size_t>(o2) * d2.m * (d2.p + 1);
+      const int e1 = d1.tab_exp[o1], e2 = d2.tab_exp[o2];
+      a.s1[gi] = ldexpf(1.f, -e1);
+      a.s2[gi] = ldexpf(1.f, -e2);
+      a.e12[gi] = e1 + e2;
+      a.out[gi] = outs[f];
+    }
     if (nin == 3)
-      a.src.tab3 = pl.dims[2].ktab32 +
-                   static_cast<size_t>(orders[f * nin + 2]) * pl.dims[2].m * (pl.dims[2].p + 1);
-    const int e1 = d1.tab_exp[o1], e2 = d2.tab_exp[o2];
-    a.s1 = ldexpf(1.f, -e1);
-    a.s2 = ldexpf(1.f, -e2);
-    a.e12 = e1 + e2;
-    a.out = outs[f];
+      a.src.tab3 = pl.dims[2].ktab32 + static_cast<size_t>(orders[group[0] * nin + 2]) *
+                                           pl.dims[2].m * (pl.dims[2].p + 1);
     for (int c0 = 0; c0 < pl.nout;) {
       const int left = pl.nout - c0;
-      const int nc = nurbs ? std::min(left, 3) : std::min(left, 4);
-      MmaKernel k = pick_mma(tier, nin, nc, nurbs);
+      const int nc = gnf > 1 ? 1 : (nurbs ? std::min(left, 3) : std::min(left, 4));
+      MmaKernel k = pick_mma(tier, nin, gnf, nc, nurbs);
       if (!k) return hipErrorInvalidValue;
       a.src.c0 = c0;
-      const size_t lds = (static_cast<size_t>(nc + (nurbs ? 1 : 0)) * tl.w1max * tl.w2max +
-                          static_cast<size_t>(tl.chunk_rows()) * (d2.p + 1)) * sizeof(float) +
-                         static_cast<size_t>(tl.chunk_rows()) * sizeof(int);
-      hipLaunchKernelGGL(k, grid, dim3(kThreads), lds, s, a);
+      hipLaunchKernelGGL(k, grid, dim3(kThreads), mma_lds(pl, nc + (nurbs ? 1 : 0), gnf), s, a);
       hipError_t e = hipGetLastError();
       if (e != hipSuccess) return e;
       c0 += nc;
