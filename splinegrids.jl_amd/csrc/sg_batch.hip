@@ -3,12 +3,16 @@
 // A batch holds one plan (FP64 tables, spans) per grid and a single list of
 // tiles over all grids -- the same tiles as fwd_stream: 64 samples of
 // dimension 1 x a chunk of rows of dimension 2 x one sample of dimension 3
-// -- sorted largest first.  A persistent kernel with a few workgroups per CU
-// takes tiles from a device work queue (one atomic counter in global memory,
-// reset by a one-thread kernel before each launch) until the queue is empty,
-// so uneven grids balance across the chip.  Degrees, channel counts, NURBS
-// weights and sizes are per tile at run time; the p2+1 register window is
-// selected by a switch over p2.
+// -- in one queue per XCD.  Grids are assigned whole to the eight queues,
+// largest first onto the least loaded one, so each grid's control points are
+// fetched into one XCD's L2 rather than into all eight.  A persistent kernel
+// with a few workgroups per CU takes tiles from the queue of its own XCD
+// (workgroups are dispatched to XCDs round-robin, so workgroup b runs on XCD
+// b mod 8) through an atomic counter in global memory, and once that queue
+// is empty from the other queues in turn, so uneven grids still balance
+// across the chip.  The counters are reset by a small kernel before each
+// launch.  Degrees, channel counts, NURBS weights and sizes are per tile at
+// run time; the p2+1 register window is selected by a switch over p2.
 #include <algorithm>
 #include <numeric>
 #include <cstring>
@@ -45,8 +49,14 @@ namespace {
 
 constexpr int kBC = 4;  // value channels per pass; the weight is slab 4
 
-__global__ void batch_reset(int* counter) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *counter = 0;
+// Queue q's counter is counters[q * kQueueStride] (one 256-byte line each);
+// its tiles are tiles[start[q] .. start[q + 1]), with start[] stored after
+// the counters.
+constexpr int kQueueStride = 64;
+constexpr int kQueueInts = kXcds * kQueueStride + kXcds + 1;
+
+__global__ void batch_reset(int* counters) {
+  if (blockIdx.x == 0 && threadIdx.x < kXcds) counters[threadIdx.x * kQueueStride] = 0;
 }
 
 // Everything a channel-block pass of a tile needs, set up once per tile.
@@ -59,51 +69,186 @@ struct TileCtx {
   float* smem;
   const float* RT;  // [chunk][P2+1] B2 rows of the chunk
   const int* RS;    // [chunk] their spans
-  int n1, p3, pm1, nch, cs, W1, lo1, lo2, r0, rb, re, j3_0, j3, planes, q1b, m1, m2;
+  const float* list;  // 3-D tiles: the plane list (build_plane_list)
+  int nk;             // ... and its length
+  int n1, p3, nch, cs, W1, lo1, lo2, r0, rb, re, j3_0, j3, planes, q1b, m1, m2;
+  unsigned cs_rcp, w1_rcp;  // udiv_rcp of cs and W1
   long long ncp, nsamp, plane_cp;
   bool active;
   int j1;
   float b1[kMaxStreamDegree + 1];
 };
 
-// One channel block [c0, c0 + NC) of a tile: stage 1 (band windows of every
-// plane in LDS), then each wave's rows.  NC and NURBS are compile-time, so the
-// row loop does exactly NC (+ weight) window sums and stores.
-template <int P2, int NC, bool NURBS>
-__device__ __attribute__((always_inline)) void tile_pass(const TileCtx& x, int c0) {
-  constexpr int PM = kMaxStreamDegree;
-  constexpr int NCH = NC + (NURBS ? 1 : 0);
-  const int cs = x.cs;
-  // ---- stage 1: band windows, contracted along dimension 3 ----
-  for (int pidx = threadIdx.x; pidx < x.planes * cs; pidx += kThreads) {
-    const int pz = pidx / cs, idx = pidx - pz * cs;
-    float* Gp = x.smem + static_cast<size_t>(pz) * x.nch * cs;
-    const int q2 = idx / x.W1, q1 = idx - q2 * x.W1;
-    const long long cpi = static_cast<long long>(x.lo1 + q1) +
-                          static_cast<long long>(x.n1) * (x.lo2 + q2);
-    float acc[NCH];
+// floor(n / d) for n * d < 2^32 from r = udiv_rcp(d): with r = ceil(2^32 / d)
+// = d r - 2^32 + e (0 <= e < d), n r / 2^32 = n / d + n e / (d 2^32), and
+// n e / 2^32 < 1 / d keeps the floor exact.  Window indices here are below
+// kMaxLds / 4 = 2^14, so n * d < 2^28.
+__host__ __device__ inline unsigned udiv_rcp(unsigned d) { return d > 1 ? 0xFFFFFFFFu / d + 1 : 0; }
+__device__ inline int udiv_fast(int n, int d, unsigned r) {
+  return d > 1 ? static_cast<int>(__umulhi(static_cast<unsigned>(n), r)) : n;
+}
+
+// Plane list of a 3-D tile: the union of the dimension-3 bands of its
+// planes, one entry per control-point plane k3 that any of them needs:
+// {w[0..3], k3}, where w[pz] is B3 of plane pz at k3 (0 outside its band).
+// Every k3 is loaded once and serves all planes.  An extra entry (zero
+// weights, a valid k3) lets stage 1 take the entries in pairs.
+constexpr int kEntry = 8;  // floats per entry: weights (16-byte aligned), k3, padding
+constexpr int kEntryK = 4;
+
+__host__ __device__ constexpr int plane_list_floats(int p3) {
+  return (kWaves * (p3 + 1) + 1) * kEntry;
+}
+
+// Builds the list into `list` (16-byte aligned LDS) and returns its length.
+// The spans of the planes are non-decreasing, so band q adds the planes
+// [max(s3[q] - p3, s3[q-1] + 1), s3[q]] to those of the bands before it.
+__device__ inline int build_plane_list(const TileCtx& x, float* list) {
+  const int p3 = x.p3;
+  int s3[kWaves], lo[kWaves], base[kWaves + 1];
 #pragma unroll
-    for (int c = 0; c < NCH; ++c) acc[c] = 0.f;
-    if (x.tab3 == nullptr) {  // 2-D grid
+  for (int q = 0; q < kWaves; ++q) s3[q] = wave_uniform(x.span3[x.j3_0 + min(q, x.planes - 1)]);
+  base[0] = 0;
+#pragma unroll
+  for (int q = 0; q < kWaves; ++q) {
+    lo[q] = q == 0 ? s3[0] - p3 : max(s3[q] - p3, s3[q - 1] + 1);
+    base[q + 1] = base[q] + max(0, s3[q] - lo[q] + 1);
+  }
+  const int nk = base[kWaves];
+  for (int t = threadIdx.x; t < (nk + 1) * kWaves; t += kThreads) {
+    const int i = t / kWaves, pz = t % kWaves;
+    int k3 = s3[kWaves - 1];  // the padding entry's plane
+#pragma unroll
+    for (int q = 0; q < kWaves; ++q)
+      if (i >= base[q] && i < base[q + 1]) k3 = lo[q] + i - base[q];
+    const int d = k3 - (s3[pz] - p3);
+    const bool in = i < nk && pz < x.planes && d >= 0 && d <= p3;
+    const float w = x.tab3[static_cast<size_t>(x.j3_0 + min(pz, x.planes - 1)) * (p3 + 1) +
+                           min(max(d, 0), p3)];
+    list[i * kEntry + pz] = in ? w : 0.f;
+    if (pz == 0) reinterpret_cast<int*>(list)[i * kEntry + kEntryK] = k3;
+  }
+  return nk;
+}
+
+// Stage 1 of a 2-D tile: the window of channels [c0, c0 + NC) (times the
+// weight, and the weight last, for NURBS) copied into LDS, U elements per
+// thread in flight.
+template <int NC, bool NURBS>
+__device__ __attribute__((always_inline)) void stage1_2d(const TileCtx& x, int c0) {
+  constexpr int NCH = NC + (NURBS ? 1 : 0);
+  constexpr int U = NCH <= 2 ? 4 : 2;
+  const int cs = x.cs;
+  for (int e0 = threadIdx.x; e0 < cs; e0 += U * kThreads) {
+    float v[U][NCH];
+    int e[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      e[u] = min(e0 + u * kThreads, cs - 1);
+      const int q2 = udiv_fast(e[u], x.W1, x.w1_rcp), q1 = e[u] - q2 * x.W1;
+      const long long cpi = static_cast<long long>(x.lo1 + q1) +
+                            static_cast<long long>(x.n1) * (x.lo2 + q2);
       const float w = NURBS ? x.W[cpi] : 1.f;
 #pragma unroll
-      for (int c = 0; c < NC; ++c) acc[c] = w * x.P[cpi + x.ncp * (c0 + c)];
-      if constexpr (NURBS) acc[NC] = w;
-    } else {
-      const int jz = x.j3_0 + pz;
-      const int s3 = x.span3[jz];
-      Global<const float>* b3 = x.tab3 + static_cast<size_t>(jz) * (x.p3 + 1);
-      for (int d = 0; d <= x.p3; ++d) {
-        const long long off = cpi + x.plane_cp * (s3 - x.p3 + d);
-        const float bw = NURBS ? b3[d] * x.W[off] : b3[d];
+      for (int c = 0; c < NC; ++c) v[u][c] = x.P[cpi + x.ncp * (c0 + c)];
+      if constexpr (NURBS) {
 #pragma unroll
-        for (int c = 0; c < NC; ++c) acc[c] += bw * x.P[off + x.ncp * (c0 + c)];
-        if constexpr (NURBS) acc[NC] += bw;
+        for (int c = 0; c < NC; ++c) v[u][c] *= w;
+        v[u][NC] = w;
       }
     }
 #pragma unroll
-    for (int c = 0; c < NCH; ++c) Gp[c * cs + idx] = acc[c];
+    for (int u = 0; u < U; ++u)
+      if (e0 + u * kThreads < cs) {
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) x.smem[c * cs + e[u]] = v[u][c];
+      }
   }
+}
+
+// Stage 1 of a 3-D tile: each window element of every plane,
+// G[pz][k2][k1] = sum over the plane list of w[pz] * (W *) P[k1, k2, k3],
+// two list entries (and two elements for up to two channels) in flight.
+template <int NC, bool NURBS>
+__device__ __attribute__((always_inline)) void stage1_3d(const TileCtx& x, int c0) {
+  typedef float f4 __attribute__((ext_vector_type(4)));
+  constexpr int NCH = NC + (NURBS ? 1 : 0);
+  constexpr int U = NCH <= 2 ? 2 : 1;
+  const int cs = x.cs, nk = x.nk;
+  const float* list = x.list;
+  for (int e0 = threadIdx.x; e0 < cs; e0 += U * kThreads) {
+    long long cpi[U];
+    int e[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      e[u] = min(e0 + u * kThreads, cs - 1);
+      const int q2 = udiv_fast(e[u], x.W1, x.w1_rcp), q1 = e[u] - q2 * x.W1;
+      cpi[u] = static_cast<long long>(x.lo1 + q1) + static_cast<long long>(x.n1) * (x.lo2 + q2);
+    }
+    float acc[U][kWaves][NCH];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int pz = 0; pz < kWaves; ++pz)
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) acc[u][pz][c] = 0.f;
+    for (int i = 0; i < nk; i += 2) {
+      float v[2][U][NCH];
+      f4 wv[2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const float* ent = list + (i + h) * kEntry;
+        wv[h] = *reinterpret_cast<const f4*>(ent);
+        const long long plane = x.plane_cp * reinterpret_cast<const int*>(ent)[kEntryK];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const long long off = cpi[u] + plane;
+          const float w = NURBS ? x.W[off] : 1.f;
+#pragma unroll
+          for (int c = 0; c < NC; ++c) v[h][u][c] = x.P[off + x.ncp * (c0 + c)];
+          if constexpr (NURBS) {
+#pragma unroll
+            for (int c = 0; c < NC; ++c) v[h][u][c] *= w;
+            v[h][u][NC] = w;
+          }
+        }
+      }
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+          for (int pz = 0; pz < kWaves; ++pz)
+#pragma unroll
+            for (int c = 0; c < NCH; ++c) acc[u][pz][c] += wv[h][pz] * v[h][u][c];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (e0 + u * kThreads < cs) {
+#pragma unroll
+        for (int pz = 0; pz < kWaves; ++pz)
+          if (pz < x.planes) {
+            float* Gp = x.smem + static_cast<size_t>(pz) * x.nch * cs;
+#pragma unroll
+            for (int c = 0; c < NCH; ++c) Gp[c * cs + e[u]] = acc[u][pz][c];
+          }
+      }
+  }
+}
+
+// One channel block [c0, c0 + NC) of a tile: stage 1 (band windows of every
+// plane in LDS), then each wave's rows.  P1, P2, NC and NURBS are
+// compile-time, so a window column is exactly P1 + 1 LDS reads and FMAs per
+// channel, and the row loop does exactly NC (+ weight) window sums and stores.
+template <int P1, int P2, int NC, bool NURBS>
+__device__ __attribute__((always_inline)) void tile_pass(const TileCtx& x, int c0) {
+  constexpr int NCH = NC + (NURBS ? 1 : 0);
+  const int cs = x.cs;
+  // ---- stage 1: band windows, contracted along dimension 3 ----
+  if (x.tab3 == nullptr)
+    stage1_2d<NC, NURBS>(x, c0);
+  else
+    stage1_3d<NC, NURBS>(x, c0);
   __syncthreads();
 
   // ---- stage 2: rows of this wave, sliding window over k2 ----
@@ -118,7 +263,7 @@ __device__ __attribute__((always_inline)) void tile_pass(const TileCtx& x, int c
       for (int ch = 0; ch < NCH; ++ch) {
         float s = 0.f;
 #pragma unroll
-        for (int q = 0; q <= PM; ++q) s += x.b1[q] * gr[ch * cs + min(q, x.pm1)];
+        for (int q = 0; q <= P1; ++q) s += x.b1[q] * gr[ch * cs + q];
         v[ch] = s;
       }
     };
@@ -166,19 +311,19 @@ __device__ __attribute__((always_inline)) void tile_pass(const TileCtx& x, int c
   __syncthreads();  // G is rewritten by the next channel block / tile
 }
 
-template <int P2, bool NURBS>
+template <int P1, int P2, bool NURBS>
 __device__ __attribute__((always_inline)) void tile_pass_nc(const TileCtx& x, int c0, int nc) {
   switch (nc) {
-    case 1: tile_pass<P2, 1, NURBS>(x, c0); break;
-    case 2: tile_pass<P2, 2, NURBS>(x, c0); break;
-    case 3: tile_pass<P2, 3, NURBS>(x, c0); break;
-    default: tile_pass<P2, 4, NURBS>(x, c0); break;
+    case 1: tile_pass<P1, P2, 1, NURBS>(x, c0); break;
+    case 2: tile_pass<P1, P2, 2, NURBS>(x, c0); break;
+    case 3: tile_pass<P1, P2, 3, NURBS>(x, c0); break;
+    default: tile_pass<P1, P2, 4, NURBS>(x, c0); break;
   }
 }
 
-template <int P2>
-__device__ void batch_tile(const BGrid& g, const BTile& t, float* smem, int lane, int wave) {
-  constexpr int PM = kMaxStreamDegree;
+template <int P1, int P2>
+__device__ __attribute__((always_inline)) void batch_tile(const BGrid& g, const BTile& t,
+                                                          float* smem, int lane, int wave) {
   // the grid's arrays as global pointers (global loads and stores)
   const auto span1 = global_ptr(g.span1), span2 = global_ptr(g.span2);
   const auto tab1 = global_ptr(g.tab1), tab2 = global_ptr(g.tab2);
@@ -186,13 +331,15 @@ __device__ void batch_tile(const BGrid& g, const BTile& t, float* smem, int lane
   const int j1_0 = t.bx * kLanes;
   const int j1_last = min(j1_0 + kLanes - 1, g.m1 - 1);
   const int r0 = t.by * chunk_cap, r1 = min(r0 + chunk_cap, g.m2);
-  const int p1 = g.p1;
+  constexpr int p1 = P1;
   TileCtx x;
   x.lo1 = span1[j1_0] - p1;
   x.W1 = span1[j1_last] - x.lo1 + 1;
   x.lo2 = span2[r0] - P2;
   const int W2 = span2[r1 - 1] - x.lo2 + 1;
   x.cs = x.W1 * W2;
+  x.cs_rcp = udiv_rcp(static_cast<unsigned>(x.cs));
+  x.w1_rcp = udiv_rcp(static_cast<unsigned>(x.W1));
   const bool nurbs = g.nurbs != 0;
   x.nch = min(kBC, g.nout) + (nurbs ? 1 : 0);
   // 3-D grids: the tile holds kWaves consecutive samples j3 (from
@@ -206,6 +353,8 @@ __device__ void batch_tile(const BGrid& g, const BTile& t, float* smem, int lane
   // host's LDS size assumes)
   float* RT = smem + static_cast<size_t>(zpack ? kWaves : 1) * x.nch * x.cs;
   int* RS = reinterpret_cast<int*>(RT + static_cast<size_t>(chunk_cap) * (P2 + 1));
+  // the plane list (3-D grids) after the spans, 16-byte aligned
+  const size_t list_at = (reinterpret_cast<float*>(RS + chunk_cap) - smem + 3) & ~size_t(3);
   {
     const size_t rbase = static_cast<size_t>(r0) * (P2 + 1);
     stage_to_lds<1, 1>(r1 - r0, [&](int, int i) { return span2[r0 + i]; },
@@ -223,7 +372,6 @@ __device__ void batch_tile(const BGrid& g, const BTile& t, float* smem, int lane
   x.out = global_ptr(g.out);
   x.n1 = g.n1;
   x.p3 = g.p3;
-  x.pm1 = p1;
   x.r0 = r0;
   x.rb = zpack ? (wave < x.planes ? r0 : r1) : r0 + wave * g.rows_per_wave;
   x.re = zpack ? r1 : min(x.rb + g.rows_per_wave, r1);
@@ -236,47 +384,76 @@ __device__ void batch_tile(const BGrid& g, const BTile& t, float* smem, int lane
   x.active = (j1_0 + lane) < g.m1;
   x.j1 = x.active ? j1_0 + lane : g.m1 - 1;
   x.q1b = span1[x.j1] - p1 - x.lo1;
-  // b1 is zero past p1, so the window sums take all PM+1 terms without a
-  // branch (the clamped index re-reads the last element of the band)
-#pragma unroll
-  for (int q = 0; q <= PM; ++q) {
-    const float v = tab1[static_cast<size_t>(x.j1) * (p1 + 1) + min(q, p1)];
-    x.b1[q] = q <= p1 ? v : 0.f;
+  x.list = smem + list_at;
+  x.nk = 0;
+  if (zpack) {  // uniform over the workgroup
+    x.nk = build_plane_list(x, smem + list_at);
+    __syncthreads();
   }
+#pragma unroll
+  for (int q = 0; q <= P1; ++q) x.b1[q] = tab1[static_cast<size_t>(x.j1) * (P1 + 1) + q];
   for (int c0 = 0; c0 < g.nout; c0 += kBC) {
     const int nc = min(kBC, g.nout - c0);
     if (nurbs)
-      tile_pass_nc<P2, true>(x, c0, nc);
+      tile_pass_nc<P1, P2, true>(x, c0, nc);
     else
-      tile_pass_nc<P2, false>(x, c0, nc);
+      tile_pass_nc<P1, P2, false>(x, c0, nc);
+  }
+}
+
+template <int P1>
+__device__ __attribute__((always_inline)) void batch_tile_p2(const BGrid& g, const BTile& t,
+                                                             float* smem, int lane, int wave) {
+  switch (g.p2) {
+    case 0: batch_tile<P1, 0>(g, t, smem, lane, wave); break;
+    case 1: batch_tile<P1, 1>(g, t, smem, lane, wave); break;
+    case 2: batch_tile<P1, 2>(g, t, smem, lane, wave); break;
+    case 3: batch_tile<P1, 3>(g, t, smem, lane, wave); break;
+    case 4: batch_tile<P1, 4>(g, t, smem, lane, wave); break;
+    default: batch_tile<P1, 5>(g, t, smem, lane, wave); break;
   }
 }
 
 __global__ __launch_bounds__(kThreads) void batch_persistent(const BGrid* __restrict__ grids,
                                                              const BTile* __restrict__ tiles,
-                                                             int n_tiles, int* counter) {
+                                                             int* counters) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   float* smem = reinterpret_cast<float*>(smem_raw);
   __shared__ int s_tile;
   const int lane = threadIdx.x & (kLanes - 1);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kLanes);
+  const int* start = counters + kXcds * kQueueStride;
+  const int home = static_cast<int>(blockIdx.x) % kXcds;
+  int visited = 0;  // thread 0: queues found empty so far
   for (;;) {
-    if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1);
+    if (threadIdx.x == 0) {
+      // own queue first, then the others; queues found empty stay skipped
+      int ti = -1;
+      for (; visited < kXcds; ++visited) {
+        const int q = (home + visited) % kXcds;
+        const int i = start[q] + atomicAdd(counters + q * kQueueStride, 1);
+        if (i < start[q + 1]) {
+          ti = i;
+          break;
+        }
+      }
+      s_tile = ti;
+    }
     __syncthreads();
     // every thread read the same value: the exit is uniform, and with the
     // index in a scalar register the tile and grid records are scalar loads
     // (branches on them stay scalar branches)
     const int ti = wave_uniform(s_tile);
-    if (ti >= n_tiles) break;
+    if (ti < 0) break;
     const BTile t = tiles[ti];
     const BGrid g = grids[wave_uniform(t.grid)];
-    switch (g.p2) {
-      case 0: batch_tile<0>(g, t, smem, lane, wave); break;
-      case 1: batch_tile<1>(g, t, smem, lane, wave); break;
-      case 2: batch_tile<2>(g, t, smem, lane, wave); break;
-      case 3: batch_tile<3>(g, t, smem, lane, wave); break;
-      case 4: batch_tile<4>(g, t, smem, lane, wave); break;
-      default: batch_tile<5>(g, t, smem, lane, wave); break;
+    switch (g.p1) {
+      case 0: batch_tile_p2<0>(g, t, smem, lane, wave); break;
+      case 1: batch_tile_p2<1>(g, t, smem, lane, wave); break;
+      case 2: batch_tile_p2<2>(g, t, smem, lane, wave); break;
+      case 3: batch_tile_p2<3>(g, t, smem, lane, wave); break;
+      case 4: batch_tile_p2<4>(g, t, smem, lane, wave); break;
+      default: batch_tile_p2<5>(g, t, smem, lane, wave); break;
     }
     // batch_tile ends with a barrier, so s_tile may be overwritten now
   }
@@ -297,7 +474,7 @@ struct sg_batch {
   std::vector<BGrid> hgrids;
   BGrid* dgrids = nullptr;
   BTile* dtiles = nullptr;
-  int* counter = nullptr;
+  int* counters = nullptr;  // kQueueInts: queue counters, then queue starts
   int n_tiles = 0;
   size_t lds = 0;
   int rows_per_wave = 16;
@@ -316,7 +493,7 @@ void free_batch(sg_batch* b) {
   for (Plan* p : b->plans) destroy_plan(p);
   if (b->dgrids) (void)hipFree(b->dgrids);
   if (b->dtiles) (void)hipFree(b->dtiles);
-  if (b->counter) (void)hipFree(b->counter);
+  if (b->counters) (void)hipFree(b->counters);
   delete b;
 }
 
@@ -342,7 +519,10 @@ GridTiling grid_tiling(const Plan& pl, int rows) {
   }
   const size_t nch = std::min(pl.nout, kBC) + (pl.nurbs ? 1 : 0);
   const size_t planes = pl.nin == 3 ? kWaves : 1;
-  t.lds = (planes * nch * t.w1 * t.w2 + static_cast<size_t>(chunk) * (d2.p + 2)) * sizeof(float);
+  // window slabs, B2 rows and spans, alignment, and the plane list of 3-D tiles
+  const size_t list = pl.nin == 3 ? 3 + plane_list_floats(pl.dims[2].p) : 0;
+  t.lds = (planes * nch * t.w1 * t.w2 + static_cast<size_t>(chunk) * (d2.p + 2) + list) *
+          sizeof(float);
   return t;
 }
 
@@ -355,14 +535,17 @@ static size_t lds_cap(int blocks_per_cu) {
   return std::min(static_cast<size_t>(kMaxLds), (share & ~size_t(511)) - 512);
 }
 
-// Tiles of every persistent grid, largest first.  Each grid takes the
+// Tiles of every persistent grid in one queue per XCD.  Each grid takes the
 // batch's rows per wave, halved until its tiles fit in the LDS share of one
 // of `blocks_per_cu` workgroups; a grid whose tiles never fit is evaluated
-// with its own launch instead.
+// with its own launch instead.  Grids go whole to the least loaded queue,
+// largest first (by outputs plus window work); a queue holds its grids in
+// that order, each grid's tiles row by row, so neighbouring tiles share
+// control-point planes in L2.
 int build_tiles(sg_batch* b, int rows_per_wave, int blocks_per_cu) {
   const size_t cap = lds_cap(blocks_per_cu);
-  std::vector<BTile> tiles;
-  std::vector<long long> cost;
+  std::vector<std::vector<BTile>> grid_tiles;
+  std::vector<long long> grid_cost;
   size_t lds = 16;
   std::vector<int> persistent;
   b->separate = b->fixed_separate;
@@ -383,6 +566,8 @@ int build_tiles(sg_batch* b, int rows_per_wave, int blocks_per_cu) {
     // 3-D grids: one tile per kWaves samples j3 (a plane per wave)
     const int m3 = pl.nin == 3 ? pl.dims[2].m : 1;
     const int nz = pl.nin == 3 ? (m3 + kWaves - 1) / kWaves : 1;
+    std::vector<BTile> tiles;
+    long long cost = 0;
     for (int z = 0; z < nz; ++z)
       for (int y = 0; y < t.nty; ++y)
         for (int x = 0; x < t.ntx; ++x) {
@@ -390,16 +575,32 @@ int build_tiles(sg_batch* b, int rows_per_wave, int blocks_per_cu) {
           const long long lanes = std::min(kLanes, d1.m - x * kLanes);
           const long long nrows = std::min(chunk, d2.m - y * chunk);
           const long long planes = pl.nin == 3 ? std::min(kWaves, m3 - z * kWaves) : 1;
-          cost.push_back(lanes * nrows * planes * pl.nout + 64LL * planes * t.w1 * t.w2);
+          cost += lanes * nrows * planes * pl.nout + 64LL * planes * t.w1 * t.w2;
         }
+    grid_tiles.push_back(std::move(tiles));
+    grid_cost.push_back(cost);
     b->hgrids[gi].rows_per_wave = rows;
   }
-  std::vector<size_t> order(tiles.size());
+  std::vector<size_t> order(grid_tiles.size());
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(),
-                   [&](size_t a, size_t c) { return cost[a] > cost[c]; });
-  std::vector<BTile> sorted(tiles.size());
-  for (size_t i = 0; i < order.size(); ++i) sorted[i] = tiles[order[i]];
+                   [&](size_t a, size_t c) { return grid_cost[a] > grid_cost[c]; });
+  std::vector<std::vector<size_t>> queue_grids(kXcds);
+  long long load[kXcds] = {};
+  for (size_t g : order) {
+    const int q = static_cast<int>(std::min_element(load, load + kXcds) - load);
+    load[q] += grid_cost[g];
+    queue_grids[q].push_back(g);
+  }
+  std::vector<BTile> sorted;
+  int host_q[kQueueInts] = {};
+  int* start = host_q + kXcds * kQueueStride;
+  for (int q = 0; q < kXcds; ++q) {
+    start[q] = static_cast<int>(sorted.size());
+    for (size_t g : queue_grids[q])
+      sorted.insert(sorted.end(), grid_tiles[g].begin(), grid_tiles[g].end());
+  }
+  start[kXcds] = static_cast<int>(sorted.size());
   if (b->dtiles) (void)hipFree(b->dtiles);
   b->dtiles = nullptr;
   hipError_t e = hipSuccess;
@@ -408,6 +609,8 @@ int build_tiles(sg_batch* b, int rows_per_wave, int blocks_per_cu) {
     if (e == hipSuccess)
       e = hipMemcpy(b->dtiles, sorted.data(), sorted.size() * sizeof(BTile), hipMemcpyHostToDevice);
   }
+  if (e == hipSuccess && b->counters)
+    e = hipMemcpy(b->counters, host_q, sizeof(host_q), hipMemcpyHostToDevice);
   if (e == hipSuccess && !b->hgrids.empty())
     e = hipMemcpy(b->dgrids, b->hgrids.data(), b->hgrids.size() * sizeof(BGrid),
                   hipMemcpyHostToDevice);
@@ -421,12 +624,13 @@ int build_tiles(sg_batch* b, int rows_per_wave, int blocks_per_cu) {
 }
 
 hipError_t launch_batch(sg_batch* b, hipStream_t s) {
-  hipLaunchKernelGGL(batch_reset, dim3(1), dim3(64), 0, s, b->counter);
+  if (b->n_tiles == 0 || !b->counters) return hipSuccess;  // nothing persistent to run
+  hipLaunchKernelGGL(batch_reset, dim3(1), dim3(64), 0, s, b->counters);
   hipError_t e = hipGetLastError();
-  if (e != hipSuccess || b->n_tiles == 0) return e;
+  if (e != hipSuccess) return e;
   const int blocks = std::max(1, std::min(b->n_tiles, b->blocks_per_cu * b->n_cu));
   hipLaunchKernelGGL(batch_persistent, dim3(blocks), dim3(kThreads), b->lds, s, b->dgrids,
-                     b->dtiles, b->n_tiles, b->counter);
+                     b->dtiles, b->counters);
   return hipGetLastError();
 }
 
@@ -505,7 +709,7 @@ int sg_batch_create(const sg_grid_desc* grids, int32_t n_grids, void* stream, sg
   }
   if (n_grids > 0) {
     e = hipMalloc(&b->dgrids, n_grids * sizeof(BGrid));
-    if (e == hipSuccess) e = hipMalloc(&b->counter, sizeof(int));
+    if (e == hipSuccess) e = hipMalloc(&b->counters, kQueueInts * sizeof(int));
     if (e != hipSuccess) return bail(hip_fail(e, "batch allocation"));
   }
   const int rc = build_tiles(b, 16, 3);

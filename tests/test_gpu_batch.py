@@ -26,6 +26,11 @@ def _batch_grids(sg, rng, n=40):
         dict(n=(9, 7), p=(0, 5), m=(70, 50), nout=2, nurbs=False),
         dict(n=(12, 9), p=(7, 2), m=(40, 30), nout=1, nurbs=False),
         dict(n=(10, 8, 6), p=(2, 3, 1), m=(33, 20, 9), nout=6, nurbs=False),
+        # 3-D plane lists: p3 = 0 and 5, fewer samples than control points
+        # along dimension 3 (disjoint bands), ragged groups of four planes,
+        # NURBS over two channel blocks
+        dict(n=(9, 12, 7), p=(1, 2, 0), m=(20, 17, 7), nout=3, nurbs=True),
+        dict(n=(8, 10, 40), p=(4, 1, 5), m=(30, 22, 13), nout=5, nurbs=True),
         dict(n=(8, 8), p=(3, 3), m=(65, 40), nout=3, nurbs=True, unsorted=True),
     ]
     return [bench.build_spec_grid(sg, spec, rng) for spec in grids]
