@@ -289,18 +289,19 @@ template <typename T>
 struct AdjGatherArgs {
   const T* part;
   T* out;
-  // per dimension d and control point k: the (block, offset) pairs of the
-  // workgroup windows that hold k, as a list [ptr[k], ptr[k+1])
+  // per dimension d and control point k: the windows that hold k, as
+  // offsets [ptr[k], ptr[k+1]) whose sum over the dimensions (plus the
+  // channel's) is the element offset in part
   const int32_t* ptr1;
-  const int2* ent1;
+  const long long* off1;
   const int32_t* ptr2;
-  const int2* ent2;
+  const long long* off2;
   const int32_t* ptr3;
-  const int2* ent3;
-  int n1, n2;
-  int nout, w1max, w2max, w3max, ntx, nty;
+  const long long* off3;
+  int n1, n2, nout;
   int accumulate;
-  long long ncp;
+  int fast;  // udiv_fast is exact for every window number of this plan
+  long long ncp, chan;
 };
 
 // Lanes per control point in adj_gather: lane `sub` of a group sums the
@@ -322,27 +323,26 @@ __global__ __launch_bounds__(256) void adj_gather(AdjGatherArgs<T> a) {
   r /= a.n1;
   const int k2 = static_cast<int>(r % a.n2);
   const int k3 = static_cast<int>(r / a.n2);
-  const size_t plane = static_cast<size_t>(a.w1max) * a.w2max;
   const int e1a = a.ptr1[k1], n1e = a.ptr1[k1 + 1] - e1a;
   const int e2a = a.ptr2[k2], n2e = a.ptr2[k2 + 1] - e2a;
   const int e3a = a.ptr3[k3], n3e = a.ptr3[k3 + 1] - e3a;
-  // the n1e * n2e * n3e windows that hold (k1, k2, k3), dimension 1 fastest
+  // the n1e * n2e * n3e windows that hold (k1, k2, k3), dimension 1 fastest:
+  // window f is (i1, i2, i3) with f = i1 + n1e (i2 + n2e i3), split with
+  // udiv_fast where the plan's counts keep it exact (a.fast, uniform)
+  const int n12 = n1e * n2e;
+  const unsigned r1 = udiv_rcp(static_cast<unsigned>(n1e)), r12 = udiv_rcp(static_cast<unsigned>(n12));
+  const long long base = static_cast<long long>(c) * a.chan;
   constexpr int kBatch = 4;
-  const int total = live ? n1e * n2e * n3e : 0;
+  const int total = live ? n12 * n3e : 0;
   T sum = T(0);
   for (int f0 = sub; f0 < total; f0 += kBatch * kGatherLanes) {
     T v[kBatch];
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) {
       const int f = min(f0 + u * kGatherLanes, total - 1);
-      const int i1 = f % n1e, rest = f / n1e;
-      const int i2 = rest % n2e, i3 = rest / n2e;
-      const int2 x = a.ent1[e1a + i1];
-      const int2 y = a.ent2[e2a + i2];
-      const int2 z = a.ent3[e3a + i3];
-      const size_t unit =
-          ((static_cast<size_t>(z.x) * a.nty + y.x) * a.ntx + x.x) * a.w3max + z.y;
-      v[u] = a.part[(unit * a.nout + c) * plane + static_cast<size_t>(y.y) * a.w1max + x.y];
+      const int i3 = a.fast ? udiv_fast(f, n12, r12) : f / n12, f12 = f - i3 * n12;
+      const int i2 = a.fast ? udiv_fast(f12, n1e, r1) : f12 / n1e, i1 = f12 - i2 * n1e;
+      v[u] = a.part[base + a.off1[e1a + i1] + a.off2[e2a + i2] + a.off3[e3a + i3]];
     }
 #pragma unroll
     for (int u = 0; u < kBatch; ++u)
@@ -530,30 +530,27 @@ hipError_t adjoint_stream(const Plan& pl, const int32_t* orders, const T* E,
     c0 += nc;
   }
 
-  return launch_adjoint_gather(pl, tl, tl.w1max, tl.w2max, pl.work, Pbar,
-                               sizeof(T) == 4 ? SG_F32 : SG_F64, accumulate, s);
+  return launch_adjoint_gather(pl, tl, pl.work, Pbar, sizeof(T) == 4 ? SG_F32 : SG_F64,
+                               accumulate, s);
 }
 
 template <typename T>
-hipError_t gather(const Plan& pl, const Tiling& tl, int w1, int w2, const T* part, T* Pbar,
-                  int accumulate, hipStream_t s) {
+hipError_t gather(const Plan& pl, const Tiling& tl, const T* part, T* Pbar, int accumulate,
+                  hipStream_t s) {
   AdjGatherArgs<T> g{};
   g.part = part;
   g.out = Pbar;
   g.ptr1 = tl.gptr[0];
-  g.ent1 = reinterpret_cast<const int2*>(tl.gent[0]);
+  g.off1 = tl.goff[0];
   g.ptr2 = tl.gptr[1];
-  g.ent2 = reinterpret_cast<const int2*>(tl.gent[1]);
+  g.off2 = tl.goff[1];
   g.ptr3 = tl.gptr[2];
-  g.ent3 = reinterpret_cast<const int2*>(tl.gent[2]);
+  g.off3 = tl.goff[2];
   g.n1 = pl.dims[0].n;
   g.n2 = pl.dims[1].n;
   g.nout = pl.nout;
-  g.w1max = w1;
-  g.w2max = w2;
-  g.w3max = tl.w3max;
-  g.ntx = tl.n_tiles1;
-  g.nty = tl.n_chunks2;
+  g.chan = tl.gchan;
+  g.fast = tl.gfast;
   g.accumulate = accumulate;
   g.ncp = 1;
   for (int d = 0; d < pl.nin; ++d) g.ncp *= pl.dims[d].n;
@@ -593,14 +590,13 @@ hipError_t adjoint_csr(const Plan& pl, const int32_t* orders, const T* E,
 
 }  // namespace
 
-hipError_t launch_adjoint_gather(const Plan& pl, const Tiling& tl, int w1, int w2,
-                                 const void* part, void* Pbar, int dtype, int accumulate,
-                                 hipStream_t s) {
+hipError_t launch_adjoint_gather(const Plan& pl, const Tiling& tl, const void* part, void* Pbar,
+                                 int dtype, int accumulate, hipStream_t s) {
   if (dtype == SG_F32)
-    return gather<float>(pl, tl, w1, w2, static_cast<const float*>(part),
-                         static_cast<float*>(Pbar), accumulate, s);
-  return gather<double>(pl, tl, w1, w2, static_cast<const double*>(part),
-                        static_cast<double*>(Pbar), accumulate, s);
+    return gather<float>(pl, tl, static_cast<const float*>(part), static_cast<float*>(Pbar),
+                         accumulate, s);
+  return gather<double>(pl, tl, static_cast<const double*>(part), static_cast<double*>(Pbar),
+                        accumulate, s);
 }
 
 size_t adjoint_work_bytes(const Plan& pl) {

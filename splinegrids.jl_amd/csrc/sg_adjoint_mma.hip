@@ -40,7 +40,7 @@ namespace {
 typedef float f4 __attribute__((ext_vector_type(4)));
 
 constexpr int kTile1 = 64;     // samples j1 per task
-constexpr int kMmaWin = 16;    // control points per MFMA tile edge
+constexpr int kMmaWin = kAdjMmaWin;  // control points per MFMA tile edge
 constexpr int kBufs = 8;  // ebar units per unrolled group; kBufs - 1 are in flight
 
 struct AdjMmaArgs {
@@ -373,7 +373,7 @@ hipError_t launch_adjoint_mma(const Plan& pl, const int32_t* orders, const float
   hipLaunchKernelGGL(k, grid, dim3(kThreads), 0, s, a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  return launch_adjoint_gather(pl, tl, kMmaWin, kMmaWin, pl.work, Pbar, SG_F32, accumulate, s);
+  return launch_adjoint_gather(pl, tl, pl.work, Pbar, SG_F32, accumulate, s);
 }
 
 }  // namespace sg

@@ -258,43 +258,59 @@ int validate_desc(const sg_grid_desc* desc) {
 
 // For every dimension d and control point k, the (block, offset) pairs of the
 // blocks of tiling tl whose band windows hold k: the lists adj_gather walks.
-static hipError_t upload_gather_lists(Plan& pl, Tiling& tl) {
-  auto lists = [&](int nblocks, int block_len, const DimDev& D, std::vector<int32_t>& ptr,
-                   std::vector<int32_t>& ent) {
-    std::vector<std::vector<int32_t>> per(D.n);
+// Gather lists of a tiling whose partial windows are w1 x w2 x w3max per
+// channel, laid out [block3][block2][block1][k3 - lo3][channel][k2 - lo2]
+// [k1 - lo1]: the element offset of control point k in a window is the sum
+// of one entry per dimension and channel * gchan.
+static hipError_t upload_gather_lists(Plan& pl, Tiling& tl, int w1, int w2) {
+  const long long plane = static_cast<long long>(w1) * w2;
+  const long long win = static_cast<long long>(tl.w3max) * pl.nout * plane;  // one block
+  // (block, offset in window) of dimension d -> its part of the element offset
+  const long long stride1 = win, stride2 = win * tl.n_tiles1,
+                  stride3 = win * tl.n_tiles1 * tl.n_chunks2;
+  auto lists = [&](int nblocks, int block_len, const DimDev& D, long long bstride,
+                   long long ostride, std::vector<int32_t>& ptr, std::vector<long long>& off) {
+    std::vector<std::vector<long long>> per(D.n);
     for (int b = 0; b < nblocks; ++b) {
       const int a = b * block_len, e = std::min(a + block_len, D.m) - 1;
       const int lo = D.h_span[a] - D.p;
-      for (int k = lo; k <= D.h_span[e]; ++k) {
-        per[k].push_back(b);
-        per[k].push_back(k - lo);
-      }
+      for (int k = lo; k <= D.h_span[e]; ++k) per[k].push_back(b * bstride + (k - lo) * ostride);
     }
     ptr.assign(D.n + 1, 0);
-    ent.clear();
+    off.clear();
     for (int k = 0; k < D.n; ++k) {
-      ent.insert(ent.end(), per[k].begin(), per[k].end());
-      ptr[k + 1] = static_cast<int32_t>(ent.size() / 2);
+      off.insert(off.end(), per[k].begin(), per[k].end());
+      ptr[k + 1] = static_cast<int32_t>(off.size());
     }
   };
-  std::vector<int32_t> ptr[3], ent[3];
-  lists(tl.n_tiles1, tl.tile_width(), pl.dims[0], ptr[0], ent[0]);
-  lists(tl.n_chunks2, tl.chunk_rows(), pl.dims[1], ptr[1], ent[1]);
+  std::vector<int32_t> ptr[3];
+  std::vector<long long> off[3];
+  lists(tl.n_tiles1, tl.tile_width(), pl.dims[0], stride1, 1, ptr[0], off[0]);
+  lists(tl.n_chunks2, tl.chunk_rows(), pl.dims[1], stride2, w1, ptr[1], off[1]);
   if (pl.nin == 3) {
-    lists(tl.n_z, tl.j3_per_block, pl.dims[2], ptr[2], ent[2]);
+    lists(tl.n_z, tl.j3_per_block, pl.dims[2], stride3, pl.nout * plane, ptr[2], off[2]);
   } else {
     ptr[2] = {0, 1};
-    ent[2] = {0, 0};
+    off[2] = {0};
   }
+  tl.gchan = plane;
+  // the gather splits a window number f < n1 n2 n3 (per-point counts) into
+  // (i1, i2, i3) with udiv_fast when f * n1 n2 and f12 * n1 stay below 2^32
+  long long nmax[3] = {1, 1, 1};
+  for (int d = 0; d < 3; ++d)
+    for (size_t k = 0; k + 1 < ptr[d].size(); ++k)
+      nmax[d] = std::max<long long>(nmax[d], ptr[d][k + 1] - ptr[d][k]);
+  const long long n12 = nmax[0] * nmax[1];
+  tl.gfast = (n12 * n12 * nmax[2] < (1LL << 32) && nmax[0] * n12 < (1LL << 32)) ? 1 : 0;
   hipError_t e = hipSuccess;
   for (int d = 0; d < 3 && e == hipSuccess; ++d) {
     e = dalloc(pl, &tl.gptr[d], ptr[d].size());
-    if (e == hipSuccess) e = dalloc(pl, &tl.gent[d], ent[d].size());
+    if (e == hipSuccess) e = dalloc(pl, &tl.goff[d], off[d].size());
     if (e == hipSuccess)
       e = hipMemcpy(tl.gptr[d], ptr[d].data(), ptr[d].size() * sizeof(int32_t),
                     hipMemcpyHostToDevice);
-    if (e == hipSuccess && !ent[d].empty())
-      e = hipMemcpy(tl.gent[d], ent[d].data(), ent[d].size() * sizeof(int32_t),
+    if (e == hipSuccess && !off[d].empty())
+      e = hipMemcpy(tl.goff[d], off[d].data(), off[d].size() * sizeof(long long),
                     hipMemcpyHostToDevice);
   }
   return e;
@@ -483,8 +499,9 @@ int build_plan(const sg_grid_desc* desc, hipStream_t s, bool want_adjoint, Plan*
     plan_adjoint_mma(*pl);
   if (ok && (pl->adj.n_tiles1 > 0 || pl->adjm.n_tiles1 > 0)) {
     hipError_t e = hipSuccess;
-    if (pl->adj.n_tiles1 > 0) e = upload_gather_lists(*pl, pl->adj);
-    if (e == hipSuccess && pl->adjm.n_tiles1 > 0) e = upload_gather_lists(*pl, pl->adjm);
+    if (pl->adj.n_tiles1 > 0) e = upload_gather_lists(*pl, pl->adj, pl->adj.w1max, pl->adj.w2max);
+    if (e == hipSuccess && pl->adjm.n_tiles1 > 0)
+      e = upload_gather_lists(*pl, pl->adjm, kAdjMmaWin, kAdjMmaWin);
     if (e == hipSuccess && pl->adjm.n_tiles1 > 0 && pl->nin == 2) {
       const float one = 1.f;
       const int32_t zero = 0;

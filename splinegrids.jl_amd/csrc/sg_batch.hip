@@ -79,14 +79,8 @@ struct TileCtx {
   float b1[kMaxStreamDegree + 1];
 };
 
-// floor(n / d) for n * d < 2^32 from r = udiv_rcp(d): with r = ceil(2^32 / d)
-// = d r - 2^32 + e (0 <= e < d), n r / 2^32 = n / d + n e / (d 2^32), and
-// n e / 2^32 < 1 / d keeps the floor exact.  Window indices here are below
-// kMaxLds / 4 = 2^14, so n * d < 2^28.
-__host__ __device__ inline unsigned udiv_rcp(unsigned d) { return d > 1 ? 0xFFFFFFFFu / d + 1 : 0; }
-__device__ inline int udiv_fast(int n, int d, unsigned r) {
-  return d > 1 ? static_cast<int>(__umulhi(static_cast<unsigned>(n), r)) : n;
-}
+// Window indices are below kMaxLds / 4 = 2^14, so udiv_fast(n, d) is exact
+// for them (n * d < 2^28).
 
 // Plane list of a 3-D tile: the union of the dimension-3 bands of its
 // planes, one entry per control-point plane k3 that any of them needs:

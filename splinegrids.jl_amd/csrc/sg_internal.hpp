@@ -20,6 +20,7 @@ constexpr int kWaves = 4;               // waves per workgroup
 constexpr int kThreads = kLanes * kWaves;
 constexpr int kMaxStreamDegree = 5;     // streaming kernels: p1, p2 <= 5
 constexpr int kMaxLds = 64 * 1024;      // LDS bytes one workgroup may claim
+constexpr int kAdjMmaWin = 16;         // matrix-core adjoint: window edge (one MFMA tile)
 
 // Workgroup copy of n >= 1 elements of NF sources into LDS:
 // store(f, i, load(f, i)) for f < NF, i < n.  Each thread issues all NF * U
@@ -57,6 +58,14 @@ __device__ inline Global<T>* global_ptr(T* p) {
 // one IEEE division.  Each sample then multiplies its channels by it.
 __device__ inline float nurbs_recip(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ inline double nurbs_recip(double x) { return 1.0 / x; }
+
+// floor(n / d) for n * d < 2^32 from r = udiv_rcp(d): with r = ceil(2^32 / d)
+// = (2^32 + e) / d (0 <= e < d), n r / 2^32 = n / d + n e / (d 2^32), and
+// n e / 2^32 < 1 / d keeps the floor exact.
+__host__ __device__ inline unsigned udiv_rcp(unsigned d) { return d > 1 ? 0xFFFFFFFFu / d + 1 : 0; }
+__device__ inline int udiv_fast(int n, int d, unsigned r) {
+  return d > 1 ? static_cast<int>(__umulhi(static_cast<unsigned>(n), r)) : n;
+}
 
 // A value that every lane of the wave holds (read from one LDS address),
 // moved to a scalar register: branches on it stay scalar branches and FMAs
@@ -101,10 +110,14 @@ struct Tiling {
   int xcd_map = 0;            // workgroup numbering, see xcd_logical_block
   // adjoint only: samples of dimension 3 per workgroup and their k3 windows
   int j3_per_block = 1, n_z = 1, w3max = 1;
-  // adjoint gather: for dimension d and control point k, the (block,
-  // offset-in-window) pairs of the workgroups whose windows hold k
+  // adjoint gather: for dimension d and control point k, the windows that
+  // hold k, [gptr[d][k], gptr[d][k + 1]) in goff[d], each as the part of the
+  // element offset in the partial buffer that dimension d contributes (the
+  // offsets of the three dimensions and the channel add up)
   int32_t* gptr[3] = {nullptr, nullptr, nullptr};  // [n_d + 1]
-  int32_t* gent[3] = {nullptr, nullptr, nullptr};  // [2 * entries]
+  long long* goff[3] = {nullptr, nullptr, nullptr};
+  long long gchan = 0;  // element offset between the channels of a window
+  int gfast = 0;        // window counts small enough for udiv_fast in the gather
   // explicit tile width / chunk height (0: derived from wx, rows_per_wave)
   int tile_w = 0, chunk_h = 0;
   int tile_width() const { return tile_w ? tile_w : kLanes * wx; }
@@ -202,7 +215,7 @@ size_t adjoint_work_bytes(const Plan& plan);
 size_t adjoint_stream_lds(const Plan& plan, int nc, size_t elem);
 // Sum the partial windows [unit][w3max][nout][w2][w1] of tiling tl into
 // Pbar (overwrite or accumulate), one thread per control point.
-hipError_t launch_adjoint_gather(const Plan& plan, const Tiling& tl, int w1, int w2,
+hipError_t launch_adjoint_gather(const Plan& plan, const Tiling& tl,
                                  const void* part, void* Pbar, int dtype, int accumulate,
                                  hipStream_t s);
 
