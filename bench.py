@@ -209,16 +209,7 @@ def fp64_errors(sg, grid, wl, outs):
     the CPU oracle.  Returns (max per field, RMS per field)."""
     import torch
 
-    dims64 = tuple(sg.SplineDimension(
-        s.n_basis_functions, s.degree, s.n_sample_points,
-        max_derivative_order=s.max_derivative_order, float_type=np.float64,
-        knot_vector=sg.KnotVector(s.knot_vector.knot_values.astype(np.float64),
-                                  s.knot_vector.multiplicities),
-        sample_points=s.host_sample_points(), device="cuda") for s in grid.spline_dimensions)
-    g64 = (sg.NURBSGrid if grid.is_nurbs else sg.SplineGrid)(dims64, grid.Nout)
-    sg.obtain(g64.control_points).copy_(sg.obtain(grid.control_points).double())
-    if grid.is_nurbs:
-        g64.weights.copy_(grid.weights.double())
+    g64 = fp64_grid(sg, grid)
     errs, rms = [], []
     ref = sg.colmajor_empty(g64.eval.shape, torch.float64, "cuda")
     for f, out in zip(wl["fields"], outs):
@@ -229,6 +220,36 @@ def fp64_errors(sg, grid, wl, outs):
         rms.append(float(d.pow(2).mean().sqrt()) / scale)
         del d
     return errs, rms
+
+
+def fp64_grid(sg, grid):
+    """This library's FP64 grid on the same knots, sample points and
+    FP32-rounded control points (and weights) promoted exactly to FP64."""
+    dims64 = tuple(sg.SplineDimension(
+        s.n_basis_functions, s.degree, s.n_sample_points,
+        max_derivative_order=s.max_derivative_order, float_type=np.float64,
+        knot_vector=sg.KnotVector(s.knot_vector.knot_values.astype(np.float64),
+                                  s.knot_vector.multiplicities),
+        sample_points=s.host_sample_points(), device="cuda") for s in grid.spline_dimensions)
+    g64 = (sg.NURBSGrid if grid.is_nurbs else sg.SplineGrid)(dims64, grid.Nout)
+    sg.obtain(g64.control_points).copy_(sg.obtain(grid.control_points).double())
+    if grid.is_nurbs:
+        g64.weights.copy_(grid.weights.double())
+    return g64
+
+
+def fp64_adjoint_errors(sg, grid, ebar, pbar):
+    """Max and RMS |Pbar - FP64 Pbar| relative to max|FP64 Pbar|, the FP64
+    adjoint taking the same ebar promoted exactly; returned as ([max], [RMS])
+    like fp64_errors' per-field lists."""
+    import torch
+
+    g64 = fp64_grid(sg, grid)
+    ref = sg.colmajor_zeros(tuple(pbar.shape), torch.float64, "cuda")
+    sg.evaluate_adjoint_(g64, eval=sg.colmajor(ebar.double()), control_points=ref)
+    d = pbar.double() - ref
+    scale = float(ref.abs().max())
+    return [float(d.abs().max()) / scale], [float(d.pow(2).mean().sqrt()) / scale]
 
 
 def traffic_from_profiles(workload_name: str):
@@ -450,25 +471,30 @@ def main(argv=None):
                      "algorithmic_bytes_per_step": B},
         "wall_s_timed": t_wall,
     }
-    if not adjoint and wl["dtype"] != np.float64:
-        # an FP64 workload would be compared with itself: no figure for it
-        out["max_rel_err_vs_fp64"], out["rms_rel_err_vs_fp64"] = fp64_errors(sg, grid, wl, outs)
-        # the other arithmetic tiers on the same inputs, timed the same way
-        if wl["dtype"] == np.float32 and args.tier == "native" and not args.no_tiers:
-            tiers = {"native": {"ms_per_step": ms, "value": value,
-                                "max_rel_err_vs_fp64": out["max_rel_err_vs_fp64"],
-                                "rms_rel_err_vs_fp64": out["rms_rel_err_vs_fp64"]}}
-            for tier in ("split_f16", "bf16"):
-                try:
-                    t_ms, _ = timed_steps(make_step(tier), max(20, args.steps // 2), args.warmup)
-                except sg.SplineGridsError as e:
-                    tiers[tier] = {"unsupported": str(e)}
-                    continue
-                mx, rms = fp64_errors(sg, grid, wl, outs)
-                tiers[tier] = {"ms_per_step": t_ms, "value": n_fields * nsamp / (t_ms * 1e-3),
-                               "max_rel_err_vs_fp64": mx, "rms_rel_err_vs_fp64": rms}
-            out["tiers"] = tiers
+    if not adjoint:
+        if wl["dtype"] != np.float64:
+            # an FP64 workload would be compared with itself: no figure for it
+            out["max_rel_err_vs_fp64"], out["rms_rel_err_vs_fp64"] = fp64_errors(sg, grid, wl, outs)
     else:
+        out["max_rel_err_vs_fp64"], out["rms_rel_err_vs_fp64"] = fp64_adjoint_errors(
+            sg, grid, ebar, pbar)
+    # the other arithmetic tiers on the same inputs, timed the same way
+    if wl["dtype"] == np.float32 and args.tier == "native" and not args.no_tiers:
+        tiers = {"native": {"ms_per_step": ms, "value": value,
+                            "max_rel_err_vs_fp64": out["max_rel_err_vs_fp64"],
+                            "rms_rel_err_vs_fp64": out["rms_rel_err_vs_fp64"]}}
+        for tier in ("split_f16", "bf16"):
+            try:
+                t_ms, _ = timed_steps(make_step(tier), max(20, args.steps // 2), args.warmup)
+            except sg.SplineGridsError as e:
+                tiers[tier] = {"unsupported": str(e)}
+                continue
+            mx, rms = (fp64_adjoint_errors(sg, grid, ebar, pbar) if adjoint
+                       else fp64_errors(sg, grid, wl, outs))
+            tiers[tier] = {"ms_per_step": t_ms, "value": n_fields * nsamp / (t_ms * 1e-3),
+                           "max_rel_err_vs_fp64": mx, "rms_rel_err_vs_fp64": rms}
+        out["tiers"] = tiers
+    if adjoint:
         # forward (evaluate!) plus backward (evaluate_adjoint!) per step
         def fwd_bwd():
             sg.evaluate_(grid)

@@ -621,7 +621,7 @@ hipError_t launch_adjoint(const Plan& pl, const int32_t* orders,
     auto e = static_cast<const float*>(ebar);
     auto o = static_cast<float*>(Pbar);
     if (pl.adjm.n_tiles1 > 0 && nsamp > 0)
-      return launch_adjoint_mma(pl, orders, e, o, accumulate, s);
+      return launch_adjoint_mma(pl, SG_TIER_NATIVE, orders, e, o, accumulate, s);
     return stream ? adjoint_stream<float>(pl, orders, e, o, accumulate, s)
                   : adjoint_csr<float>(pl, orders, e, o, accumulate, s);
   }

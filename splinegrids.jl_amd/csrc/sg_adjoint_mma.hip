@@ -31,7 +31,7 @@
 // registers for three waves per SIMD; a wave keeps kBufs - 1 row blocks of
 // ebar in flight ahead of the one it multiplies, across row blocks,
 // samples j3 and channels.
-#include "sg_internal.hpp"
+#include "sg_mma_ops.hpp"
 
 namespace sg {
 
@@ -71,7 +71,7 @@ __device__ inline int j1_of(int g, int s) { return 16 * (s >> 2) + 4 * g + (s & 
 constexpr unsigned kOff = 0x80000000u;   // always past a plane (< 2 GiB)
 constexpr int kRsrcWord3 = 0x00020000;   // raw buffer, 32-bit elements (gfx9)
 
-template <int P3, int NB, bool VEC>
+template <int P3, int NB, bool VEC, int TIER>
 __global__ __launch_bounds__(kThreads) void adj_mma(AdjMmaArgs a) {
   static_assert(kBufs % NB == 0, "row block of a buffer position must be constant");
   const int lane = threadIdx.x & (kLanes - 1);
@@ -109,6 +109,27 @@ __global__ __launch_bounds__(kThreads) void adj_mma(AdjMmaArgs a) {
                              min(max(off, 0), p1)];
       b1[s] = v * ((j1 < a.m1 && off >= 0 && off <= p1) ? 1.f : 0.f);
     }
+  }
+  // MFMA tiers: GEMM1 on 16x16x16 FP16 / BF16 products.  K block q of a
+  // unit holds samples j1_0 + 16 q + 0..15, lane (g, i) the four samples
+  // 16 q + 4 g .. + 3 -- the same samples as its 16-byte load q and as
+  // K-steps 4 q .. 4 q + 3 of the FP32 path.  B1 is scaled by 2^-eb (split
+  // FP16: max|B1| of the task below 1) and converted once per task.
+  Operand<TIER == SG_TIER_NATIVE ? SG_TIER_BF16 : TIER> b1op[4];
+  int eb = 0;
+  if constexpr (TIER != SG_TIER_NATIVE) {
+    float sb = 1.f;
+    if constexpr (TIER == SG_TIER_SPLIT_F16) {
+      float m = 0.f;
+#pragma unroll
+      for (int s = 0; s < 16; ++s) m = fmaxf(m, fabsf(b1[s]));
+      eb = scale_exponent(wave_max(m));
+      sb = ldexpf(1.f, -eb);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) b1op[q].set(t, b1[4 * q + t] * sb);
   }
   // GEMM2 A operand of row block b, K-step r: B2 at (r0 + 16 b + 4 g + r, lo2 + i)
   float b2[NB][4];
@@ -220,9 +241,36 @@ __global__ __launch_bounds__(kThreads) void adj_mma(AdjMmaArgs a) {
       if (u < U) {
         const int b = k % NB;
         f4 d1 = {0.f, 0.f, 0.f, 0.f};
+        if constexpr (TIER == SG_TIER_NATIVE) {
 #pragma unroll
-        for (int s = 0; s < 16; ++s)
-          d1 = __builtin_amdgcn_mfma_f32_16x16x4f32(buf[k][s >> 2][s & 3], b1[s], d1, 0, 0, 0);
+          for (int s = 0; s < 16; ++s)
+            d1 = __builtin_amdgcn_mfma_f32_16x16x4f32(buf[k][s >> 2][s & 3], b1[s], d1, 0, 0, 0);
+        } else {
+          // split FP16: the unit's ebar scaled by 2^-eu (max below 1), and
+          // T1 scaled back by 2^(eu + eb); BF16 needs no scaling
+          int eu = 0;
+          float su = 1.f;
+          if constexpr (TIER == SG_TIER_SPLIT_F16) {
+            float m = 0.f;
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+              for (int t = 0; t < 4; ++t) m = fmaxf(m, fabsf(buf[k][q][t]));
+            eu = scale_exponent(wave_max(m));
+            su = ldexpf(1.f, -eu);
+          }
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            Operand<TIER> op;
+#pragma unroll
+            for (int t = 0; t < 4; ++t) op.set(t, buf[k][q][t] * su);
+            d1 = Operand<TIER>::mma(op, b1op[q], d1);
+          }
+          if constexpr (TIER == SG_TIER_SPLIT_F16) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) d1[r] = ldexpf(d1[r], eu + eb);
+          }
+        }
 #pragma unroll
         for (int r = 0; r < 4; ++r)
           acc = __builtin_amdgcn_mfma_f32_16x16x4f32(b2[b][r], d1[r], acc, 0, 0, 0);
@@ -269,29 +317,39 @@ __global__ __launch_bounds__(kThreads) void adj_mma(AdjMmaArgs a) {
 
 using AdjMmaKernel = void (*)(AdjMmaArgs);
 
-template <int P3, int NB>
-AdjMmaKernel pick_vec(bool vec) {
-  return vec ? adj_mma<P3, NB, true> : adj_mma<P3, NB, false>;
-}
-
-template <int P3>
-AdjMmaKernel pick_nb(int nb, bool vec) {
-  switch (nb) {
-    case 1: return pick_vec<P3, 1>(vec);
-    case 2: return pick_vec<P3, 2>(vec);
-    case 4: return pick_vec<P3, 4>(vec);
+template <int P3, int NB, bool VEC>
+AdjMmaKernel pick_tier(int tier) {
+  switch (tier) {
+    case SG_TIER_NATIVE: return adj_mma<P3, NB, VEC, SG_TIER_NATIVE>;
+    case SG_TIER_SPLIT_F16: return adj_mma<P3, NB, VEC, SG_TIER_SPLIT_F16>;
+    case SG_TIER_BF16: return adj_mma<P3, NB, VEC, SG_TIER_BF16>;
   }
   return nullptr;
 }
 
-AdjMmaKernel pick_adj_mma(int p3, int nb, bool vec) {
+template <int P3, int NB>
+AdjMmaKernel pick_vec(bool vec, int tier) {
+  return vec ? pick_tier<P3, NB, true>(tier) : pick_tier<P3, NB, false>(tier);
+}
+
+template <int P3>
+AdjMmaKernel pick_nb(int nb, bool vec, int tier) {
+  switch (nb) {
+    case 1: return pick_vec<P3, 1>(vec, tier);
+    case 2: return pick_vec<P3, 2>(vec, tier);
+    case 4: return pick_vec<P3, 4>(vec, tier);
+  }
+  return nullptr;
+}
+
+AdjMmaKernel pick_adj_mma(int p3, int nb, bool vec, int tier) {
   switch (p3) {
-    case 0: return pick_nb<0>(nb, vec);
-    case 1: return pick_nb<1>(nb, vec);
-    case 2: return pick_nb<2>(nb, vec);
-    case 3: return pick_nb<3>(nb, vec);
-    case 4: return pick_nb<4>(nb, vec);
-    case 5: return pick_nb<5>(nb, vec);
+    case 0: return pick_nb<0>(nb, vec, tier);
+    case 1: return pick_nb<1>(nb, vec, tier);
+    case 2: return pick_nb<2>(nb, vec, tier);
+    case 3: return pick_nb<3>(nb, vec, tier);
+    case 4: return pick_nb<4>(nb, vec, tier);
+    case 5: return pick_nb<5>(nb, vec, tier);
   }
   return nullptr;
 }
@@ -309,7 +367,7 @@ bool adjoint_mma_shape_ok(const Plan& pl) {
 }
 
 int adjoint_mma_resident_waves(const Plan& pl) {
-  AdjMmaKernel k = pick_adj_mma(plan_p3(pl), pl.adjm.chunk_rows() / 16, true);
+  AdjMmaKernel k = pick_adj_mma(plan_p3(pl), pl.adjm.chunk_rows() / 16, true, SG_TIER_NATIVE);
   if (!k) return 0;
   int blocks = 0, cus = 0, dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return 0;
@@ -328,7 +386,7 @@ size_t adjoint_mma_work_bytes(const Plan& pl) {
   return units * t.w3max * pl.nout * kMmaWin * kMmaWin * sizeof(float);
 }
 
-hipError_t launch_adjoint_mma(const Plan& pl, const int32_t* orders, const float* E,
+hipError_t launch_adjoint_mma(const Plan& pl, int tier, const int32_t* orders, const float* E,
                               float* Pbar, int accumulate, hipStream_t s) {
   const Tiling& tl = pl.adjm;
   const DimDev& d1 = pl.dims[0];
@@ -366,7 +424,7 @@ hipError_t launch_adjoint_mma(const Plan& pl, const int32_t* orders, const float
   // 16-byte loads need 16-byte rows: m1 % 4 == 0 and an aligned base
   const bool vec = (d1.m % 4 == 0) && (reinterpret_cast<uintptr_t>(E) % 16 == 0);
   const dim3 grid(static_cast<unsigned>((a.ntasks + kWaves - 1) / kWaves));
-  AdjMmaKernel k = pick_adj_mma(plan_p3(pl), tl.chunk_rows() / 16, vec);
+  AdjMmaKernel k = pick_adj_mma(plan_p3(pl), tl.chunk_rows() / 16, vec, tier);
   if (!k) return hipErrorInvalidValue;
   a.c0 = 0;
   a.nc = pl.nout;

@@ -722,14 +722,23 @@ int sg_evaluate_adjoint(sg_plan* plan, const int32_t* derivative_order, const vo
   if (rc != SG_OK) return rc;
   const Plan& pl = plan->p;
   if (pl.nurbs) return fail(SG_ERR_INVALID, "Adjoint evaluation not supported for NURBS.");
-  if (tier != SG_TIER_NATIVE)
-    return fail(SG_ERR_UNSUPPORTED, "the adjoint implements the native tier only");
+  if (tier != SG_TIER_NATIVE && pl.adjm.n_tiles1 == 0)
+    return fail(SG_ERR_UNSUPPORTED,
+                "the split-FP16 and BF16 adjoint tiers run on the matrix-core adjoint, which "
+                "needs band windows of at most 16 control points per 64 samples of dimension 1 "
+                "and per 16..64 samples of dimension 2 (and SG_ADJ_MMA not 0)");
   if ((rc = check_orders(pl, 1, derivative_order)) != SG_OK) return rc;
   if (n_control(pl) == 0) return SG_OK;
   if (!control_points || (!eval && n_samples(pl) > 0))
     return fail(SG_ERR_INVALID, "null device pointer");
   DeviceGuard guard(pl.device);
   if (guard.err != hipSuccess) return hip_fail(guard.err, "hipSetDevice");
+  if (tier != SG_TIER_NATIVE && n_samples(pl) > 0)
+    return checked(launch_adjoint_mma(pl, tier, derivative_order,
+                                      static_cast<const float*>(eval),
+                                      static_cast<float*>(control_points), accumulate ? 1 : 0,
+                                      static_cast<hipStream_t>(stream)),
+                   "adjoint launch");
   return checked(launch_adjoint(pl, derivative_order, eval, control_points, dtype,
                                 accumulate ? 1 : 0, static_cast<hipStream_t>(stream)),
                  "adjoint launch");

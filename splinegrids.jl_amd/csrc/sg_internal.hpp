@@ -223,7 +223,7 @@ hipError_t launch_adjoint_gather(const Plan& plan, const Tiling& tl,
 bool adjoint_mma_shape_ok(const Plan& plan);
 int adjoint_mma_resident_waves(const Plan& plan);  // on the current device
 size_t adjoint_mma_work_bytes(const Plan& plan);
-hipError_t launch_adjoint_mma(const Plan& plan, const int32_t* orders, const float* ebar,
+hipError_t launch_adjoint_mma(const Plan& plan, int tier, const int32_t* orders, const float* ebar,
                               float* Pbar, int accumulate, hipStream_t s);
 
 }  // namespace sg

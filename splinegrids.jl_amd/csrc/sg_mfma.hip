@@ -19,15 +19,12 @@
 // lane l = 16 g + i holds A[i][4g..4g+3], B[4g..4g+3][i], D[4g..4g+3][i].
 #include <vector>
 
+#include "sg_mma_ops.hpp"
 #include "sg_tiles.hpp"
 
 namespace sg {
 
 namespace {
-
-typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
-typedef short short4_t __attribute__((ext_vector_type(4)));
-typedef float float4_t __attribute__((ext_vector_type(4)));
 
 struct MmaArgs {
   BandSrc<float> src;
@@ -44,40 +41,6 @@ struct MmaArgs {
   float s2[SG_MAX_FIELDS];
   int e12[SG_MAX_FIELDS];        // e1 + e2 of each field
   long long nsamp;
-};
-
-__device__ inline short bf16_bits(float x) {  // round to nearest even
-  unsigned u = __float_as_uint(x);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return static_cast<short>(u >> 16);
-}
-
-template <int TIER>
-struct Operand;
-
-template <>
-struct Operand<SG_TIER_SPLIT_F16> {
-  half4_t hi, lo;
-  __device__ void set(int j, float x) {
-    const _Float16 h = static_cast<_Float16>(x);
-    hi[j] = h;
-    lo[j] = static_cast<_Float16>(x - static_cast<float>(h));
-  }
-  __device__ static float4_t mma(const Operand& a, const Operand& b, float4_t c) {
-    c = __builtin_amdgcn_mfma_f32_16x16x16f16(a.hi, b.hi, c, 0, 0, 0);
-    c = __builtin_amdgcn_mfma_f32_16x16x16f16(a.hi, b.lo, c, 0, 0, 0);
-    c = __builtin_amdgcn_mfma_f32_16x16x16f16(a.lo, b.hi, c, 0, 0, 0);
-    return c;
-  }
-};
-
-template <>
-struct Operand<SG_TIER_BF16> {
-  short4_t v;
-  __device__ void set(int j, float x) { v[j] = bf16_bits(x); }
-  __device__ static float4_t mma(const Operand& a, const Operand& b, float4_t c) {
-    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a.v, b.v, c, 0, 0, 0);
-  }
 };
 
 template <int TIER, int NIN, int NF, int NC, bool NURBS>

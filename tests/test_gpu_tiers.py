@@ -85,9 +85,11 @@ def test_tier_rejections(gpu):
     grid = sg.SplineGrid(tuple(dims), 1)
     with pytest.raises(sg.SplineGridsError, match="FP32 arrays"):
         sg.evaluate_(grid, tier="split_f16")
-    dims = make_dims(sg, (6, 5), (2, 2), (20, 20))
+    # 64 samples of dimension 1 over 60 control points: windows wider than
+    # 16, so no matrix-core adjoint and no adjoint tiers
+    dims = make_dims(sg, (60, 8), (3, 2), (64, 30))
     grid = sg.SplineGrid(tuple(dims), 1)
-    with pytest.raises(sg.SplineGridsError, match="native tier only"):
+    with pytest.raises(sg.SplineGridsError, match="matrix-core adjoint"):
         sg.evaluate_adjoint_(grid, tier="split_f16")
     d1 = make_dims(sg, (6,), (2,), (20,))
     with pytest.raises(sg.SplineGridsError, match="Nin 2-3"):
