@@ -110,12 +110,13 @@ __global__ __launch_bounds__(kThreads) void adj_mma(AdjMmaArgs a) {
       b1[s] = v * ((j1 < a.m1 && off >= 0 && off <= p1) ? 1.f : 0.f);
     }
   }
-  // MFMA tiers: GEMM1 on 16x16x16 FP16 / BF16 products.  K block q of a
-  // unit holds samples j1_0 + 16 q + 0..15, lane (g, i) the four samples
-  // 16 q + 4 g .. + 3 -- the same samples as its 16-byte load q and as
-  // K-steps 4 q .. 4 q + 3 of the FP32 path.  B1 is scaled by 2^-eb (split
-  // FP16: max|B1| of the task below 1) and converted once per task.
-  Operand<TIER == SG_TIER_NATIVE ? SG_TIER_BF16 : TIER> b1op[4];
+  // MFMA tiers: GEMM1 on 16x16x32 FP16 / BF16 products.  K block h of a
+  // unit holds samples j1_0 + 32 h + 0..31; element e of lane (g, i) is
+  // sample 16 (2 h + e / 4) + 4 g + e % 4 in both operands -- element e % 4
+  // of its 16-byte load 2 h + e / 4, and K-step 8 h + e of the FP32 path.
+  // B1 is scaled by 2^-eb (split FP16: max|B1| of the task below 1) and
+  // converted once per task.
+  Operand8<TIER == SG_TIER_NATIVE ? SG_TIER_BF16 : TIER> b1op[2];
   int eb = 0;
   if constexpr (TIER != SG_TIER_NATIVE) {
     float sb = 1.f;
@@ -127,9 +128,9 @@ __global__ __launch_bounds__(kThreads) void adj_mma(AdjMmaArgs a) {
       sb = ldexpf(1.f, -eb);
     }
 #pragma unroll
-    for (int q = 0; q < 4; ++q)
+    for (int h = 0; h < 2; ++h)
 #pragma unroll
-      for (int t = 0; t < 4; ++t) b1op[q].set(t, b1[4 * q + t] * sb);
+      for (int e = 0; e < 8; ++e) b1op[h].set(e, b1[8 * h + e] * sb);
   }
   // GEMM2 A operand of row block b, K-step r: B2 at (r0 + 16 b + 4 g + r, lo2 + i)
   float b2[NB][4];
@@ -260,11 +261,11 @@ __global__ __launch_bounds__(kThreads) void adj_mma(AdjMmaArgs a) {
             su = ldexpf(1.f, -eu);
           }
 #pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            Operand<TIER> op;
+          for (int h = 0; h < 2; ++h) {
+            Operand8<TIER> op;
 #pragma unroll
-            for (int t = 0; t < 4; ++t) op.set(t, buf[k][q][t] * su);
-            d1 = Operand<TIER>::mma(op, b1op[q], d1);
+            for (int e = 0; e < 8; ++e) op.set(e, buf[k][2 * h + e / 4][e % 4] * su);
+            d1 = Operand8<TIER>::mma(op, b1op[h], d1);
           }
           if constexpr (TIER == SG_TIER_SPLIT_F16) {
 #pragma unroll
