@@ -137,7 +137,9 @@ int sg_evaluate_fields(sg_plan* plan, int32_t n_fields,
  * control_points without atomics; the result is deterministic.
  * accumulate = 0 overwrites control_points (the reference zeroes it first,
  * src/adjoint.jl:61); accumulate = 1 adds to it (reverse-mode AD).
- * B-spline plans only (src/adjoint.jl:57). */
+ * B-spline plans only (src/adjoint.jl:57).  The split-FP16 and BF16 tiers
+ * run on the matrix-core adjoint (FP32 plans whose band windows fit 16 x 16
+ * control points per tile); other plans return SG_ERR_UNSUPPORTED for them. */
 int sg_evaluate_adjoint(sg_plan* plan, const int32_t* derivative_order,
                         const void* eval, void* control_points, int32_t dtype,
                         int32_t tier, int32_t accumulate, void* stream);
