@@ -215,7 +215,7 @@ def fp64_errors(sg, grid, wl, outs):
     for f, out in zip(wl["fields"], outs):
         sg.evaluate_(g64, derivative_order=f, eval=ref)
         d = out.double() - ref
-        scale = float(ref.abs().max())
+        scale = float(ref.abs().max()) or 1.0  # an all-zero field: absolute error
         errs.append(float(d.abs().max()) / scale)
         rms.append(float(d.pow(2).mean().sqrt()) / scale)
         del d
@@ -248,7 +248,7 @@ def fp64_adjoint_errors(sg, grid, ebar, pbar):
     ref = sg.colmajor_zeros(tuple(pbar.shape), torch.float64, "cuda")
     sg.evaluate_adjoint_(g64, eval=sg.colmajor(ebar.double()), control_points=ref)
     d = pbar.double() - ref
-    scale = float(ref.abs().max())
+    scale = float(ref.abs().max()) or 1.0
     return [float(d.abs().max()) / scale], [float(d.pow(2).mean().sqrt()) / scale]
 
 

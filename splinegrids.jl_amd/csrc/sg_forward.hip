@@ -126,29 +126,34 @@ __global__ __launch_bounds__(kThreads) void fwd_stream(
 #pragma unroll
   for (int q = 0; q <= P2; ++q) column(cur - P2 + q, V[q]);
 
-  // span and B2 rows of row j2 (scalar loads), fetched one row ahead
-  auto row_data = [&](int j2, int& s2, T (&bb)[NF][P2 + 1]) {
-    s2 = span2[j2];
+  // Span and B2 rows of each row come from scalar loads issued one row
+  // ahead.  Rows go in pairs with two register sets, A and B, so that no
+  // prefetched value is copied, and the scalar pointers advance by one row
+  // (they stay on the last row at the end instead of being recomputed from a
+  // clamped index).
+  const int32_t* sp = span2 + rb;
+  const T* tp[NF];
+#pragma unroll
+  for (int f = 0; f < NF; ++f) tp[f] = tab2[f] + static_cast<size_t>(rb) * (P2 + 1);
+  auto fetch = [&](int& s2, T (&bb)[NF][P2 + 1]) {
+    s2 = *sp;
 #pragma unroll
     for (int f = 0; f < NF; ++f)
 #pragma unroll
-      for (int q = 0; q <= P2; ++q) bb[f][q] = tab2[f][static_cast<size_t>(j2) * (P2 + 1) + q];
+      for (int q = 0; q <= P2; ++q) bb[f][q] = tp[f][q];
   };
-  int s_next;
-  T b_next[NF][P2 + 1];
-  row_data(rb, s_next, b_next);
+  auto advance = [&](bool more) {
+    if (more) {
+      ++sp;
+#pragma unroll
+      for (int f = 0; f < NF; ++f) tp[f] += P2 + 1;
+    }
+  };
 
-  const long long obase = static_cast<long long>(j1) +
-                          static_cast<long long>(a.m1) * a.m2 * j3 +
-                          a.nsamp * a.c0;
-  for (int j2 = rb; j2 < re; ++j2) {
-    const int s2 = s_next;
-    T bb[NF][P2 + 1];
-#pragma unroll
-    for (int f = 0; f < NF; ++f)
-#pragma unroll
-      for (int q = 0; q <= P2; ++q) bb[f][q] = b_next[f][q];
-    row_data(min(j2 + 1, re - 1), s_next, b_next);
+  long long orow = static_cast<long long>(j1) + static_cast<long long>(a.m1) * a.m2 * j3 +
+                   a.nsamp * a.c0 + static_cast<long long>(a.m1) * rb;
+  // one row: bring the window to span s2, then every field's outputs
+  auto row = [&](int s2, const T (&bb)[NF][P2 + 1]) {
     if (s2 != cur) {
       if (s2 - cur > P2) {
         cur = s2;
@@ -167,7 +172,6 @@ __global__ __launch_bounds__(kThreads) void fwd_stream(
         }
       }
     }
-    const long long orow = obase + static_cast<long long>(a.m1) * j2;
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
       T num[NCH];
@@ -189,7 +193,22 @@ __global__ __launch_bounds__(kThreads) void fwd_stream(
           *dst = v;
       }
     }
+    orow += a.m1;
+  };
+
+  int sA, sB;
+  T bA[NF][P2 + 1], bB[NF][P2 + 1];
+  fetch(sA, bA);  // row rb
+  int j2 = rb;
+  for (; j2 + 1 < re; j2 += 2) {
+    advance(true);
+    fetch(sB, bB);  // row j2 + 1
+    row(sA, bA);
+    advance(j2 + 2 < re);
+    fetch(sA, bA);  // row j2 + 2 (row j2 + 1 again after the last pair)
+    row(sB, bB);
   }
+  if (j2 < re) row(sA, bA);
 }
 
 // ---- direct kernel (general case) ----
