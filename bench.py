@@ -383,7 +383,7 @@ def bench_batch(args, sg, wl):
                    "setup_s": setup_s},
         "roofline": {"bound": "hbm", "achieved": B / (ms_graph * 1e-3) / 1e9,
                      "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": B / (ms_graph * 1e-3) / 1e9 / PEAK_HBM_GBS, "traffic": None,
+                     "frac": B / (ms_graph * 1e-3) / 1e9 / PEAK_HBM_GBS, "traffic": traffic_from_profiles(args.workload),
                      "algorithmic_bytes_per_step": B},
         "per_grid_loop_ms": ms_loop, "batched_ms": ms_batch, "graph_ms": ms_graph,
         "speedup_vs_per_grid_loop": ms_loop / ms_graph,
