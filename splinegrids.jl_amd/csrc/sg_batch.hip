@@ -67,11 +67,11 @@ struct TileCtx {
   Global<const float>* W;
   Global<float>* out;
   float* smem;
-  const float* RT;  // [chunk][P2+1] B2 rows of the chunk
-  const int* RS;    // [chunk] their spans
+  Constant<const float>* tab2;    // [m2][P2+1] B2 rows, read with scalar loads
+  Constant<const int32_t>* span2;  // [m2] their spans
   const float* list;  // 3-D tiles: the plane list (build_plane_list)
   int nk;             // ... and its length
-  int n1, p3, nch, cs, W1, lo1, lo2, r0, rb, re, j3_0, j3, planes, q1b, m1, m2;
+  int n1, p3, nch, cs, W1, lo1, lo2, rb, re, j3_0, j3, planes, q1b, m1, m2;
   unsigned cs_rcp, w1_rcp;  // udiv_rcp of cs and W1
   long long ncp, nsamp, plane_cp;
   bool active;
@@ -261,13 +261,31 @@ __device__ __attribute__((always_inline)) void tile_pass(const TileCtx& x, int c
         v[ch] = s;
       }
     };
-    int cur = wave_uniform(x.RS[x.rb - x.r0]);
+    // Span and B2 row of each row from scalar loads issued one row ahead,
+    // rows in pairs with two register sets (no copies of prefetched values);
+    // the pointers stay on the last row at the end.
+    Constant<const int32_t>* sp = x.span2 + x.rb;
+    Constant<const float>* tp = x.tab2 + static_cast<size_t>(x.rb) * (P2 + 1);
+    auto fetch = [&](int& s2, float (&bb)[P2 + 1]) {
+      s2 = *sp;
+#pragma unroll
+      for (int q = 0; q <= P2; ++q) bb[q] = tp[q];
+    };
+    auto advance = [&](bool more) {
+      if (more) {
+        ++sp;
+        tp += P2 + 1;
+      }
+    };
+    int sA, sB;
+    float bA[P2 + 1], bB[P2 + 1];
+    fetch(sA, bA);  // row rb
+    int cur = sA;
 #pragma unroll
     for (int q = 0; q <= P2; ++q) column(cur - P2 + q, V[q]);
-    const long long obase = static_cast<long long>(x.j1) +
-                            static_cast<long long>(x.m1) * x.m2 * x.j3 + x.nsamp * c0;
-    for (int j2 = x.rb; j2 < x.re; ++j2) {
-      const int s2 = wave_uniform(x.RS[j2 - x.r0]);
+    long long o = static_cast<long long>(x.j1) + static_cast<long long>(x.m1) * x.m2 * x.j3 +
+                  x.nsamp * c0 + static_cast<long long>(x.m1) * x.rb;
+    auto row = [&](int s2, const float (&bb)[P2 + 1]) {
       if (s2 != cur) {
         if (s2 - cur > P2) {
           cur = s2;
@@ -284,10 +302,6 @@ __device__ __attribute__((always_inline)) void tile_pass(const TileCtx& x, int c
           }
         }
       }
-      const float* b2p = x.RT + static_cast<size_t>(j2 - x.r0) * (P2 + 1);
-      float bb[P2 + 1];
-#pragma unroll
-      for (int q = 0; q <= P2; ++q) bb[q] = wave_uniform(b2p[q]);
       float num[NCH];
 #pragma unroll
       for (int ch = 0; ch < NCH; ++ch) {
@@ -296,11 +310,21 @@ __device__ __attribute__((always_inline)) void tile_pass(const TileCtx& x, int c
         for (int q = 0; q <= P2; ++q) s += bb[q] * V[q][ch];
         num[ch] = s;
       }
-      const long long o = obase + static_cast<long long>(x.m1) * j2;
       const float inv = NURBS ? nurbs_recip(num[NC]) : 1.f;
 #pragma unroll
       for (int c = 0; c < NC; ++c) x.out[o + x.nsamp * c] = NURBS ? num[c] * inv : num[c];
+      o += x.m1;
+    };
+    int j2 = x.rb;
+    for (; j2 + 1 < x.re; j2 += 2) {
+      advance(true);
+      fetch(sB, bB);  // row j2 + 1
+      row(sA, bA);
+      advance(j2 + 2 < x.re);
+      fetch(sA, bA);  // row j2 + 2 (row j2 + 1 again after the last pair)
+      row(sB, bB);
     }
+    if (j2 < x.re) row(sA, bA);
   }
   __syncthreads();  // G is rewritten by the next channel block / tile
 }
@@ -320,7 +344,7 @@ __device__ __attribute__((always_inline)) void batch_tile(const BGrid& g, const 
                                                           float* smem, int lane, int wave) {
   // the grid's arrays as global pointers (global loads and stores)
   const auto span1 = global_ptr(g.span1), span2 = global_ptr(g.span2);
-  const auto tab1 = global_ptr(g.tab1), tab2 = global_ptr(g.tab2);
+  const auto tab1 = global_ptr(g.tab1);
   const int chunk_cap = g.rows_per_wave * kWaves;
   const int j1_0 = t.bx * kLanes;
   const int j1_last = min(j1_0 + kLanes - 1, g.m1 - 1);
@@ -342,23 +366,14 @@ __device__ __attribute__((always_inline)) void batch_tile(const BGrid& g, const 
   const bool zpack = g.nin == 3;
   x.j3_0 = zpack ? t.bz * kWaves : 0;
   x.planes = zpack ? min(kWaves, g.m3 - x.j3_0) : 1;
-  // [planes][nch][cs] window slabs (weight last), then the chunk's B2 rows
-  // and spans (space for kWaves planes is reserved for 3-D grids, as the
-  // host's LDS size assumes)
-  float* RT = smem + static_cast<size_t>(zpack ? kWaves : 1) * x.nch * x.cs;
-  int* RS = reinterpret_cast<int*>(RT + static_cast<size_t>(chunk_cap) * (P2 + 1));
-  // the plane list (3-D grids) after the spans, 16-byte aligned
-  const size_t list_at = (reinterpret_cast<float*>(RS + chunk_cap) - smem + 3) & ~size_t(3);
-  {
-    const size_t rbase = static_cast<size_t>(r0) * (P2 + 1);
-    stage_to_lds<1, 1>(r1 - r0, [&](int, int i) { return span2[r0 + i]; },
-                       [&](int, int i, int v) { RS[i] = v; });
-    stage_to_lds<1, 4>((r1 - r0) * (P2 + 1), [&](int, int i) { return tab2[rbase + i]; },
-                       [&](int, int i, float v) { RT[i] = v; });
-  }
+  // [planes][nch][cs] window slabs (weight last; space for kWaves planes is
+  // reserved for 3-D grids, as the host's LDS size assumes), then the plane
+  // list of 3-D grids, 16-byte aligned
+  const size_t list_at =
+      (static_cast<size_t>(zpack ? kWaves : 1) * x.nch * x.cs + 3) & ~size_t(3);
   x.smem = smem;
-  x.RT = RT;
-  x.RS = RS;
+  x.tab2 = constant_ptr(g.tab2);
+  x.span2 = constant_ptr(g.span2);
   x.span3 = zpack ? global_ptr(g.span3) : nullptr;
   x.tab3 = zpack ? global_ptr(g.tab3) : nullptr;
   x.P = global_ptr(g.P);
@@ -366,7 +381,6 @@ __device__ __attribute__((always_inline)) void batch_tile(const BGrid& g, const 
   x.out = global_ptr(g.out);
   x.n1 = g.n1;
   x.p3 = g.p3;
-  x.r0 = r0;
   x.rb = zpack ? (wave < x.planes ? r0 : r1) : r0 + wave * g.rows_per_wave;
   x.re = zpack ? r1 : min(x.rb + g.rows_per_wave, r1);
   x.j3 = x.j3_0 + (zpack ? wave : 0);
@@ -513,10 +527,9 @@ GridTiling grid_tiling(const Plan& pl, int rows) {
   }
   const size_t nch = std::min(pl.nout, kBC) + (pl.nurbs ? 1 : 0);
   const size_t planes = pl.nin == 3 ? kWaves : 1;
-  // window slabs, B2 rows and spans, alignment, and the plane list of 3-D tiles
+  // window slabs, alignment and the plane list of 3-D tiles
   const size_t list = pl.nin == 3 ? 3 + plane_list_floats(pl.dims[2].p) : 0;
-  t.lds = (planes * nch * t.w1 * t.w2 + static_cast<size_t>(chunk) * (d2.p + 2) + list) *
-          sizeof(float);
+  t.lds = (planes * nch * t.w1 * t.w2 + list) * sizeof(float);
   return t;
 }
 

@@ -53,6 +53,17 @@ __device__ inline Global<T>* global_ptr(T* p) {
   return (Global<T>*)p;
 }
 
+// A pointer into memory that no kernel writes while it runs, typed as the
+// constant address space: loads at wave-uniform addresses through it become
+// scalar loads (s_load), which the compiler cannot prove safe for a plain
+// pointer read from memory.
+template <typename T>
+using Constant = __attribute__((address_space(4))) T;
+template <typename T>
+__device__ inline Constant<T>* constant_ptr(T* p) {
+  return (Constant<T>*)p;
+}
+
 // 1/x for the NURBS quotient: FP32 takes the hardware reciprocal (at most
 // 1 ulp from the correctly rounded value; 1/0 = inf, so 0/0 stays NaN), FP64
 // one IEEE division.  Each sample then multiplies its channels by it.
