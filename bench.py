@@ -408,8 +408,8 @@ def main(argv=None):
     args = ap.parse_args(argv)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: the evaluation shards by sample slabs with no "
-              "communication, so a multi-GPU figure is N independent single-GPU runs "
-              "(see DESIGN.md); use --gpus 1.", file=sys.stderr)
+              "communication, so a multi-GPU figure is N independent single-GPU runs; "
+              "use --gpus 1.", file=sys.stderr)
         return 2
 
     import torch

@@ -4,10 +4,10 @@
  * This is the drop-in boundary for the hot path of SplineGrids.jl.  The
  * reference reaches its kernels through KernelAbstractions' backend dispatch
  * (backend = get_backend(eval); kernel(backend)(...; ndrange); synchronize):
- * src/spline_grid.jl:216-228, src/adjoint.jl:300-311,
- * src/spline_dimension.jl:236-240, src/utils.jl:240-248.  A Julia package
+ * src/spline_grid.jl:216-228, src/adjoint.jl:70-81,
+ * src/spline_dimension.jl:236-240, src/utils.jl:20-28.  A Julia package
  * extension replaces those launches with ccall-s into the functions below
- * (the binding is written out in INTEGRATION.md).
+ * (the Julia binding is given with the integration notes).
  *
  * Conventions
  *   - Plain C types only; every function returns SG_OK (0) or an SG_ERR_*
@@ -57,7 +57,7 @@ typedef struct sg_plan sg_plan;
 /* One input dimension.  Mirrors SplineDimension (src/spline_dimension.jl:
  * 25-66): degree, max_derivative_order, knot_vector.knots_all and
  * sample_points.  Host pointers, FP64 values (Float32 values promote
- * exactly).  n_basis must equal n_knots - degree - 1 (src/utils.jl:252-254). */
+ * exactly).  n_basis must equal n_knots - degree - 1 (src/utils.jl:31-33). */
 typedef struct {
   int32_t degree;
   int32_t n_basis;
@@ -82,7 +82,7 @@ const char* sg_last_error(void);
 int sg_version(int32_t* major, int32_t* minor, int32_t* patch);
 
 /* set_sample_indices! + evaluate!(::SplineDimension) on caller arrays.
- * Replaces src/utils.jl:239-249 (set_sample_indices_kernel,
+ * Replaces src/utils.jl:19-29 (set_sample_indices_kernel,
  * src/util_kernels.jl:22-49) and src/spline_dimension.jl:231-242
  * (spline_dimension_kernel, :160-215).  Device pointers:
  *   knots_all[n_knots], sample_points[n_samples]    (dtype)

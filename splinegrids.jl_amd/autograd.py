@@ -8,7 +8,7 @@ map control_points -> eval is linear, so the rules are exact:
   forward:  d(eval) = M dP     (evaluate_ on the tangent)
 
 Basis tables and knots are constants.  B-spline grids only in reverse mode,
-as evaluate_adjoint! (src/adjoint.jl:287).
+as evaluate_adjoint! (src/adjoint.jl:57).
 """
 from __future__ import annotations
 

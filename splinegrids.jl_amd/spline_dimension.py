@@ -67,7 +67,7 @@ class SplineDimension:
         self.eval = colmajor_zeros((m, w, nd), self.float_type, self.device)
         evaluate_dimension_(self)
 
-    # ---- size helpers, src/utils.jl:251-276 ----
+    # ---- size helpers, src/utils.jl:31-56 ----
     @property
     def n_basis_functions(self) -> int:
         return len(self.knot_vector.knots_all) - self.degree - 1
@@ -91,7 +91,7 @@ class SplineDimension:
 
 def evaluate_dimension_(s: SplineDimension) -> None:
     """set_sample_indices! + evaluate!(::SplineDimension)
-    (src/utils.jl:239-249, src/spline_dimension.jl:231-242) on the GPU."""
+    (src/utils.jl:19-29, src/spline_dimension.jl:231-242) on the GPU."""
     L = _lib.lib()
     with torch.cuda.device(s.device):
         _lib.check(L.sg_dimension_evaluate(
