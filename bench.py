@@ -333,7 +333,7 @@ def timed_steps(step, steps: int, warmup: int):
 
 
 def bench_batch(args, sg, wl):
-    """C5: the per-grid launch loop, the batched persistent launch, and the
+    """The 1024-grid batch: the per-grid launch loop, the batched persistent launch, and the
     batched launch captured in a hipGraph; value is the graph replay."""
     import torch
 

@@ -1,9 +1,9 @@
-"""The benchmark configurations at full size.
+"""The benchmark workloads at full size (bench.py's c1..c5).
 
 The oracle checks random samples of every field (the full direct sum would
 take minutes on the host); size-independent properties cover the rest:
 partition of unity, unit weights = B-spline, the transpose identity, central
-finite differences of the C4 least-squares loss, determinism.
+finite differences of the 400 x 400 x 200 least-squares loss, determinism.
 """
 from __future__ import annotations
 
