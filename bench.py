@@ -222,6 +222,19 @@ def fp64_errors(sg, grid, wl, outs):
     return errs, rms
 
 
+def batch_fp64_errors(grids, refs):
+    """([max over grids of max|eval - FP64| / max|FP64|], [RMS over all samples
+    of |eval - FP64| / max|FP64| of the sample's grid])."""
+    worst, sq, count = 0.0, 0.0, 0
+    for g, ref in zip(grids, refs):
+        scale = float(ref.abs().max()) or 1.0
+        d = (g.eval.double() - ref).abs() / scale
+        worst = max(worst, float(d.max()))
+        sq += float(d.pow(2).sum())
+        count += d.numel()
+    return [worst], [math.sqrt(sq / max(count, 1))]
+
+
 def fp64_grid(sg, grid):
     """This library's FP64 grid on the same knots, sample points and
     FP32-rounded control points (and weights) promoted exactly to FP64."""
@@ -373,6 +386,27 @@ def bench_batch(args, sg, wl):
     ms_graph = timed(graph.replay, steps)
     nsamp = info["samples"]
     B = batch_bytes(grids)
+    # errors of every grid against this library's FP64 path on the same inputs,
+    # for the native batch (the graph replays' result) and the MFMA tiers
+    refs = []
+    for g in grids:
+        g64 = fp64_grid(sg, g)
+        sg.evaluate_(g64)
+        refs.append(g64.eval)
+    errs = batch_fp64_errors(grids, refs)
+    tiers = {"native": {"ms_per_step": ms_graph, "value": nsamp / (ms_graph * 1e-3),
+                        "max_rel_err_vs_fp64": errs[0], "rms_rel_err_vs_fp64": errs[1]}}
+    if not args.no_tiers:
+        for tier in ("split_f16", "bf16"):
+            try:
+                t_ms = timed(lambda: batch.evaluate_(tier=tier), steps)
+            except sg.SplineGridsError as e:
+                tiers[tier] = {"unsupported": str(e)}
+                continue
+            torch.cuda.synchronize()
+            e_max, e_rms = batch_fp64_errors(grids, refs)
+            tiers[tier] = {"ms_per_step": t_ms, "value": nsamp / (t_ms * 1e-3),
+                           "max_rel_err_vs_fp64": e_max, "rms_rel_err_vs_fp64": e_rms}
     out = {
         "metric": METRIC, "value": nsamp / (ms_graph * 1e-3), "unit": "sample-pts/s",
         "n_gpus": 1, "steps": steps, "warmup": args.warmup, "ms_per_step": ms_graph,
@@ -387,6 +421,7 @@ def bench_batch(args, sg, wl):
                      "algorithmic_bytes_per_step": B},
         "per_grid_loop_ms": ms_loop, "batched_ms": ms_batch, "graph_ms": ms_graph,
         "speedup_vs_per_grid_loop": ms_loop / ms_graph,
+        "max_rel_err_vs_fp64": errs[0], "rms_rel_err_vs_fp64": errs[1], "tiers": tiers,
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -19,6 +19,7 @@
 #include <sstream>
 #include <vector>
 
+#include "sg_queue.hpp"
 #include "sg_tiles.hpp"
 
 namespace sg {
@@ -41,23 +42,11 @@ struct BGrid {
   long long ncp, nsamp;
 };
 
-struct BTile {
-  int grid, bx, by, bz;  // bz: samples j3 from kWaves * bz (3-D grids)
-};
 
 namespace {
 
 constexpr int kBC = 4;  // value channels per pass; the weight is slab 4
 
-// Queue q's counter is counters[q * kQueueStride] (one 256-byte line each);
-// its tiles are tiles[start[q] .. start[q + 1]), with start[] stored after
-// the counters.
-constexpr int kQueueStride = 64;
-constexpr int kQueueInts = kXcds * kQueueStride + kXcds + 1;
-
-__global__ void batch_reset(int* counters) {
-  if (blockIdx.x == 0 && threadIdx.x < kXcds) counters[threadIdx.x * kQueueStride] = 0;
-}
 
 // Everything a channel-block pass of a tile needs, set up once per tile.
 struct TileCtx {
@@ -430,23 +419,10 @@ __global__ __launch_bounds__(kThreads) void batch_persistent(const BGrid* __rest
   __shared__ int s_tile;
   const int lane = threadIdx.x & (kLanes - 1);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kLanes);
-  const int* start = counters + kXcds * kQueueStride;
   const int home = static_cast<int>(blockIdx.x) % kXcds;
   int visited = 0;  // thread 0: queues found empty so far
   for (;;) {
-    if (threadIdx.x == 0) {
-      // own queue first, then the others; queues found empty stay skipped
-      int ti = -1;
-      for (; visited < kXcds; ++visited) {
-        const int q = (home + visited) % kXcds;
-        const int i = start[q] + atomicAdd(counters + q * kQueueStride, 1);
-        if (i < start[q + 1]) {
-          ti = i;
-          break;
-        }
-      }
-      s_tile = ti;
-    }
+    if (threadIdx.x == 0) s_tile = pop_tile(counters, home, visited);
     __syncthreads();
     // every thread read the same value: the exit is uniform, and with the
     // index in a scalar register the tile and grid records are scalar loads
@@ -483,6 +459,16 @@ struct sg_batch {
   BGrid* dgrids = nullptr;
   BTile* dtiles = nullptr;
   int* counters = nullptr;  // kQueueInts: queue counters, then queue starts
+  // MFMA tiers: one argument record per (grid, channel block) and their tiles
+  // in queues of their own; mma_ok when every grid takes the tiers
+  bool mma_ok = false;
+  std::vector<unsigned char> hmargs;
+  std::vector<int> mrec_grid, mrec_c0, mrec_nc;
+  void* dmargs = nullptr;
+  BTile* dtiles_m = nullptr;
+  int* counters_m = nullptr;
+  int n_tiles_m = 0;
+  size_t lds_m = 0;
   int n_tiles = 0;
   size_t lds = 0;
   int rows_per_wave = 16;
@@ -502,6 +488,9 @@ void free_batch(sg_batch* b) {
   if (b->dgrids) (void)hipFree(b->dgrids);
   if (b->dtiles) (void)hipFree(b->dtiles);
   if (b->counters) (void)hipFree(b->counters);
+  if (b->dmargs) (void)hipFree(b->dmargs);
+  if (b->dtiles_m) (void)hipFree(b->dtiles_m);
+  if (b->counters_m) (void)hipFree(b->counters_m);
   delete b;
 }
 
@@ -540,6 +529,94 @@ constexpr size_t kLdsPerCu = 160 * 1024;
 static size_t lds_cap(int blocks_per_cu) {
   const size_t share = kLdsPerCu / static_cast<size_t>(std::max(1, blocks_per_cu));
   return std::min(static_cast<size_t>(kMaxLds), (share & ~size_t(511)) - 512);
+}
+
+// The queues of a tile list: grids go whole to the least loaded queue,
+// largest first; a queue holds its grids in that order.  Writes the tiles in
+// queue order and the queue starts (host_q: zero counters, then starts).
+void pack_queues(const std::vector<std::vector<BTile>>& grid_tiles,
+                 const std::vector<long long>& grid_cost, std::vector<BTile>& sorted,
+                 int* host_q) {
+  std::vector<size_t> order(grid_tiles.size());
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](size_t a, size_t c) { return grid_cost[a] > grid_cost[c]; });
+  std::vector<std::vector<size_t>> queue_grids(kXcds);
+  long long load[kXcds] = {};
+  for (size_t g : order) {
+    const int q = static_cast<int>(std::min_element(load, load + kXcds) - load);
+    load[q] += grid_cost[g];
+    queue_grids[q].push_back(g);
+  }
+  sorted.clear();
+  std::fill(host_q, host_q + kQueueInts, 0);
+  int* start = host_q + kXcds * kQueueStride;
+  for (int q = 0; q < kXcds; ++q) {
+    start[q] = static_cast<int>(sorted.size());
+    for (size_t g : queue_grids[q])
+      sorted.insert(sorted.end(), grid_tiles[g].begin(), grid_tiles[g].end());
+  }
+  start[kXcds] = static_cast<int>(sorted.size());
+}
+
+// MFMA tiers: argument records and tiles of every grid (the grids' arrays
+// are filled into the records by sg_batch_set_arrays).  Leaves mma_ok false
+// when a grid cannot take the tiers.
+int build_mma_tiles(sg_batch* b) {
+  b->mma_ok = false;
+  for (Plan* pl : b->plans)
+    if (!mma_supported(*pl)) return SG_OK;
+  const size_t rec = mma_args_size();
+  std::vector<std::vector<BTile>> grid_tiles;
+  std::vector<long long> grid_cost;
+  size_t lds = 16;
+  for (size_t gi = 0; gi < b->plans.size(); ++gi) {
+    const Plan& pl = *b->plans[gi];
+    const Tiling& t = pl.mma;
+    const int m1 = pl.dims[0].m, m2 = pl.dims[1].m;
+    const int m3 = pl.nin == 3 ? pl.dims[2].m : 1;
+    const int chunk = t.chunk_rows();
+    std::vector<BTile> tiles;
+    long long cost = 0;
+    for (int c0 = 0; c0 < pl.nout;) {
+      const int nc = std::min(pl.nout - c0, pl.nurbs ? 3 : 4);
+      const int r = static_cast<int>(b->mrec_grid.size());
+      b->mrec_grid.push_back(static_cast<int>(gi));
+      b->mrec_c0.push_back(c0);
+      b->mrec_nc.push_back(nc);
+      lds = std::max(lds, mma_lds(pl, nc + (pl.nurbs ? 1 : 0), 1));
+      for (int z = 0; z < m3; ++z)
+        for (int y = 0; y < t.n_chunks2; ++y)
+          for (int x = 0; x < t.n_tiles1; ++x) {
+            tiles.push_back(BTile{r, x, y, z});
+            cost += static_cast<long long>(std::min(kLanes, m1 - x * kLanes)) *
+                    std::min(chunk, m2 - y * chunk) * nc;
+          }
+      c0 += nc;
+    }
+    grid_tiles.push_back(std::move(tiles));
+    grid_cost.push_back(cost);
+  }
+  std::vector<BTile> sorted;
+  int host_q[kQueueInts] = {};
+  pack_queues(grid_tiles, grid_cost, sorted, host_q);
+  b->hmargs.assign(b->mrec_grid.size() * rec, 0);
+  hipError_t e = hipSuccess;
+  if (!sorted.empty()) {
+    e = hipMalloc(&b->dtiles_m, sorted.size() * sizeof(BTile));
+    if (e == hipSuccess)
+      e = hipMemcpy(b->dtiles_m, sorted.data(), sorted.size() * sizeof(BTile),
+                    hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc(&b->counters_m, kQueueInts * sizeof(int));
+    if (e == hipSuccess)
+      e = hipMemcpy(b->counters_m, host_q, sizeof(host_q), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc(&b->dmargs, b->hmargs.size());
+  }
+  if (e != hipSuccess) return hip_fail(e, "batch MFMA tile upload");
+  b->n_tiles_m = static_cast<int>(sorted.size());
+  b->lds_m = lds;
+  b->mma_ok = true;
+  return SG_OK;
 }
 
 // Tiles of every persistent grid in one queue per XCD.  Each grid takes the
@@ -588,26 +665,9 @@ int build_tiles(sg_batch* b, int rows_per_wave, int blocks_per_cu) {
     grid_cost.push_back(cost);
     b->hgrids[gi].rows_per_wave = rows;
   }
-  std::vector<size_t> order(grid_tiles.size());
-  std::iota(order.begin(), order.end(), 0);
-  std::stable_sort(order.begin(), order.end(),
-                   [&](size_t a, size_t c) { return grid_cost[a] > grid_cost[c]; });
-  std::vector<std::vector<size_t>> queue_grids(kXcds);
-  long long load[kXcds] = {};
-  for (size_t g : order) {
-    const int q = static_cast<int>(std::min_element(load, load + kXcds) - load);
-    load[q] += grid_cost[g];
-    queue_grids[q].push_back(g);
-  }
   std::vector<BTile> sorted;
   int host_q[kQueueInts] = {};
-  int* start = host_q + kXcds * kQueueStride;
-  for (int q = 0; q < kXcds; ++q) {
-    start[q] = static_cast<int>(sorted.size());
-    for (size_t g : queue_grids[q])
-      sorted.insert(sorted.end(), grid_tiles[g].begin(), grid_tiles[g].end());
-  }
-  start[kXcds] = static_cast<int>(sorted.size());
+  pack_queues(grid_tiles, grid_cost, sorted, host_q);
   if (b->dtiles) (void)hipFree(b->dtiles);
   b->dtiles = nullptr;
   hipError_t e = hipSuccess;
@@ -632,7 +692,7 @@ int build_tiles(sg_batch* b, int rows_per_wave, int blocks_per_cu) {
 
 hipError_t launch_batch(sg_batch* b, hipStream_t s) {
   if (b->n_tiles == 0 || !b->counters) return hipSuccess;  // nothing persistent to run
-  hipLaunchKernelGGL(batch_reset, dim3(1), dim3(64), 0, s, b->counters);
+  hipLaunchKernelGGL(queue_reset, dim3(1), dim3(64), 0, s, b->counters);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   const int blocks = std::max(1, std::min(b->n_tiles, b->blocks_per_cu * b->n_cu));
@@ -719,8 +779,9 @@ int sg_batch_create(const sg_grid_desc* grids, int32_t n_grids, void* stream, sg
     if (e == hipSuccess) e = hipMalloc(&b->counters, kQueueInts * sizeof(int));
     if (e != hipSuccess) return bail(hip_fail(e, "batch allocation"));
   }
-  const int rc = build_tiles(b, 16, 3);
+  int rc = build_tiles(b, 16, 3);
   if (rc != SG_OK) return bail(rc);
+  if (n_grids > 0 && (rc = build_mma_tiles(b)) != SG_OK) return bail(rc);
   b->P.assign(n_grids, nullptr);
   b->W.assign(n_grids, nullptr);
   b->out.assign(n_grids, nullptr);
@@ -767,14 +828,28 @@ int sg_batch_set_arrays(sg_batch* b, const void* const* control_points, const vo
   hipError_t e = hipSuccess;
   if (n > 0)
     e = hipMemcpy(b->dgrids, b->hgrids.data(), n * sizeof(BGrid), hipMemcpyHostToDevice);
+  if (e == hipSuccess && b->mma_ok && b->dmargs) {
+    const size_t rec = mma_args_size();
+    for (size_t r = 0; r < b->mrec_grid.size(); ++r) {
+      const int gi = b->mrec_grid[r];
+      mma_fill_args(*b->plans[gi], b->mrec_c0[r], b->mrec_nc[r], b->hgrids[gi].P,
+                    b->plans[gi]->nurbs ? b->hgrids[gi].W : nullptr, b->hgrids[gi].out,
+                    b->hmargs.data() + r * rec);
+    }
+    e = hipMemcpy(b->dmargs, b->hmargs.data(), b->hmargs.size(), hipMemcpyHostToDevice);
+  }
   if (prev != b->device) (void)hipSetDevice(prev);
   return e == hipSuccess ? SG_OK : hip_fail(e, "batch array upload");
 }
 
 int sg_batch_evaluate(sg_batch* b, int32_t tier, void* stream) {
   if (!b) return fail(SG_ERR_INVALID, "batch is NULL");
-  if (tier != SG_TIER_NATIVE)
-    return fail(SG_ERR_UNSUPPORTED, "batches implement the native tier only");
+  if (tier != SG_TIER_NATIVE && tier != SG_TIER_SPLIT_F16 && tier != SG_TIER_BF16)
+    return fail(SG_ERR_INVALID, "unknown tier");
+  if (tier != SG_TIER_NATIVE && !b->mma_ok)
+    return fail(SG_ERR_UNSUPPORTED,
+                "the split-FP16 and BF16 tiers need every grid of the batch to take them "
+                "(Nin 2-3, monotone sample points, degrees <= 5 in dimensions 1-2)");
   for (size_t i = 0; i < b->out.size(); ++i)
     if (!b->out[i] && b->hgrids[i].nsamp > 0)
       return fail(SG_ERR_INVALID, "sg_batch_set_arrays was not called");
@@ -782,6 +857,16 @@ int sg_batch_evaluate(sg_batch* b, int32_t tier, void* stream) {
   (void)hipGetDevice(&prev);
   if (prev != b->device) (void)hipSetDevice(b->device);
   hipStream_t s = static_cast<hipStream_t>(stream);
+  if (tier != SG_TIER_NATIVE) {
+    hipError_t e = hipSuccess;
+    if (b->n_tiles_m > 0) {
+      const int per_cu = std::max(1, batch_mma_resident(tier, b->lds_m));
+      const int blocks = std::max(1, std::min(b->n_tiles_m, per_cu * b->n_cu));
+      e = launch_batch_mma(tier, b->dmargs, b->dtiles_m, b->counters_m, blocks, b->lds_m, s);
+    }
+    if (prev != b->device) (void)hipSetDevice(prev);
+    return e == hipSuccess ? SG_OK : hip_fail(e, "batch MFMA launch");
+  }
   hipError_t e = launch_batch(b, s);
   const int32_t zero[SG_MAX_NIN] = {0, 0, 0, 0, 0, 0, 0, 0};
   for (size_t k = 0; e == hipSuccess && k < b->separate.size(); ++k) {
@@ -882,7 +967,8 @@ int sg_batch_describe(const sg_batch* b, char* text, int32_t capacity) {
      << " separate=" << b->separate.size() << " tiles=" << b->n_tiles
      << " samples=" << b->n_samples << " rows_per_wave=" << b->rows_per_wave
      << " blocks_per_cu=" << b->blocks_per_cu << " resident_per_cu=" << resident
-     << " lds_bytes=" << b->lds << " cus=" << b->n_cu;
+     << " lds_bytes=" << b->lds << " cus=" << b->n_cu << " mfma_tiers=" << (b->mma_ok ? 1 : 0)
+     << " mfma_tiles=" << b->n_tiles_m << " mfma_lds_bytes=" << b->lds_m;
   const std::string s = os.str();
   const size_t n = std::min(s.size(), static_cast<size_t>(capacity - 1));
   std::memcpy(text, s.data(), n);

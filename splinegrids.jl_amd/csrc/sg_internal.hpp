@@ -217,6 +217,14 @@ size_t mma_lds(const Plan& plan, int nch, int nf);
 hipError_t launch_forward_mma(const Plan& plan, int tier, int nf, const int32_t* orders,
                               const float* P, const float* W, float* const* outs,
                               hipStream_t s);
+// Batched MFMA tiers: argument records (one per grid channel block), their
+// occupancy and the persistent launch over the work queues (sg_queue.hpp).
+size_t mma_args_size();
+void mma_fill_args(const Plan& plan, int c0, int nc, const float* P, const float* W, float* out,
+                   void* record);
+int batch_mma_resident(int tier, size_t lds);
+hipError_t launch_batch_mma(int tier, const void* args, const void* tiles, int* counters,
+                            int blocks, size_t lds, hipStream_t s);
 
 // ---- sg_adjoint.hip ----
 hipError_t launch_adjoint(const Plan& plan, const int32_t* orders,

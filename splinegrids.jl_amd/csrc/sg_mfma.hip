@@ -17,9 +17,11 @@
 // where B2d is the banded B2 made dense on the fly.  V^T is kept while the
 // rows' bands stay inside [kb, kb+16).  MFMA operand layout (gfx9 16x16x16):
 // lane l = 16 g + i holds A[i][4g..4g+3], B[4g..4g+3][i], D[4g..4g+3][i].
+#include <cstring>
 #include <vector>
 
 #include "sg_mma_ops.hpp"
+#include "sg_queue.hpp"
 #include "sg_tiles.hpp"
 
 namespace sg {
@@ -41,11 +43,15 @@ struct MmaArgs {
   float s2[SG_MAX_FIELDS];
   int e12[SG_MAX_FIELDS];        // e1 + e2 of each field
   long long nsamp;
+  int nin, nc, nurbs;            // batched launches: what the record needs
 };
 
+// One tile (64 samples j1 x a chunk of rows x sample j3) of channels
+// [src.c0, src.c0 + NC): stage 1, the chunk's B2 rows, the window's scale,
+// then each wave's 16 samples j1 over the chunk's rows in blocks of 16.
 template <int TIER, int NIN, int NF, int NC, bool NURBS>
-__global__ __launch_bounds__(kThreads) void mma_stream(MmaArgs a) {
-  extern __shared__ __align__(16) unsigned char smem[];
+__device__ __attribute__((always_inline)) void mma_tile(const MmaArgs& a, const FwdTile& tile,
+                                                        unsigned char* smem) {
   constexpr int NCH = NC + (NURBS ? 1 : 0);
   constexpr int PM = kMaxStreamDegree;
   float* G = reinterpret_cast<float*>(smem);
@@ -53,8 +59,6 @@ __global__ __launch_bounds__(kThreads) void mma_stream(MmaArgs a) {
 
   const int lane = threadIdx.x & (kLanes - 1);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kLanes);
-  const FwdTile tile = fwd_tile(a.xcd_map, a.ntx, a.nty, kLanes, a.rows_per_chunk, a.m1, a.m2,
-                                NIN);
   const int r0 = tile.r0, r1 = tile.r1, j3 = tile.j3;
   const int p1 = a.p1, p2 = a.p2;
   const int lo1 = a.span1[tile.j1_0] - p1;
@@ -202,6 +206,80 @@ __global__ __launch_bounds__(kThreads) void mma_stream(MmaArgs a) {
   }
 }
 
+template <int TIER, int NIN, int NF, int NC, bool NURBS>
+__global__ __launch_bounds__(kThreads) void mma_stream(MmaArgs a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const FwdTile tile = fwd_tile(a.xcd_map, a.ntx, a.nty, kLanes, a.rows_per_chunk, a.m1, a.m2,
+                                NIN);
+  mma_tile<TIER, NIN, NF, NC, NURBS>(a, tile, smem);
+}
+
+// Batched launch of the MFMA tiers: a persistent kernel takes tiles of many
+// grids from the work queues; each tile names the argument record of one
+// grid's channel block (field: derivative order 0), the tile along
+// dimension 1, the chunk along dimension 2 and the sample j3.
+template <int TIER>
+__device__ __attribute__((always_inline)) void batch_mma_dispatch(const MmaArgs& a, const FwdTile& tile,
+                                                                  unsigned char* smem) {
+  if (a.nin == 2) {
+    if (a.nurbs) {
+      switch (a.nc) {
+        case 1: mma_tile<TIER, 2, 1, 1, true>(a, tile, smem); break;
+        case 2: mma_tile<TIER, 2, 1, 2, true>(a, tile, smem); break;
+        default: mma_tile<TIER, 2, 1, 3, true>(a, tile, smem); break;
+      }
+    } else {
+      switch (a.nc) {
+        case 1: mma_tile<TIER, 2, 1, 1, false>(a, tile, smem); break;
+        case 2: mma_tile<TIER, 2, 1, 2, false>(a, tile, smem); break;
+        case 3: mma_tile<TIER, 2, 1, 3, false>(a, tile, smem); break;
+        default: mma_tile<TIER, 2, 1, 4, false>(a, tile, smem); break;
+      }
+    }
+  } else {
+    if (a.nurbs) {
+      switch (a.nc) {
+        case 1: mma_tile<TIER, 3, 1, 1, true>(a, tile, smem); break;
+        case 2: mma_tile<TIER, 3, 1, 2, true>(a, tile, smem); break;
+        default: mma_tile<TIER, 3, 1, 3, true>(a, tile, smem); break;
+      }
+    } else {
+      switch (a.nc) {
+        case 1: mma_tile<TIER, 3, 1, 1, false>(a, tile, smem); break;
+        case 2: mma_tile<TIER, 3, 1, 2, false>(a, tile, smem); break;
+        case 3: mma_tile<TIER, 3, 1, 3, false>(a, tile, smem); break;
+        default: mma_tile<TIER, 3, 1, 4, false>(a, tile, smem); break;
+      }
+    }
+  }
+}
+
+template <int TIER>
+__global__ __launch_bounds__(kThreads) void batch_mma(const MmaArgs* __restrict__ args,
+                                                      const BTile* __restrict__ tiles,
+                                                      int* counters) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ int s_tile;
+  const int home = static_cast<int>(blockIdx.x) % kXcds;
+  int visited = 0;  // thread 0: queues found empty so far
+  for (;;) {
+    if (threadIdx.x == 0) s_tile = pop_tile(counters, home, visited);
+    __syncthreads();
+    const int ti = wave_uniform(s_tile);  // the same in every thread: a uniform exit
+    if (ti < 0) break;
+    const BTile t = tiles[ti];
+    const MmaArgs a = args[wave_uniform(t.grid)];
+    FwdTile tile;
+    tile.j1_0 = t.bx * kLanes;
+    tile.j1_last = min(tile.j1_0 + kLanes - 1, a.m1 - 1);
+    tile.r0 = t.by * a.rows_per_chunk;
+    tile.r1 = min(tile.r0 + a.rows_per_chunk, a.m2);
+    tile.j3 = t.bz;
+    batch_mma_dispatch<TIER>(a, tile, smem);
+    __syncthreads();  // LDS and s_tile are rewritten for the next tile
+  }
+}
+
 using MmaKernel = void (*)(MmaArgs);
 
 template <int TIER, int NIN>
@@ -330,6 +408,83 @@ hipError_t launch_forward_mma(const Plan& pl, int tier, int nf, const int32_t* o
     }
   }
   return hipSuccess;
+}
+
+size_t mma_args_size() { return sizeof(MmaArgs); }
+
+// The argument record of channels [c0, c0 + nc) of a plan's order-0 field
+// for the batched launch (P, W, out: the grid's arrays).
+void mma_fill_args(const Plan& pl, int c0, int nc, const float* P, const float* W, float* out,
+                   void* record) {
+  const DimDev& d1 = pl.dims[0];
+  const DimDev& d2 = pl.dims[1];
+  MmaArgs a{};
+  a.src.P = P;
+  a.src.W = W;
+  a.src.n1 = d1.n;
+  a.src.n2 = d2.n;
+  a.src.c0 = c0;
+  a.span1 = d1.span;
+  a.span2 = d2.span;
+  a.m1 = d1.m;
+  a.m2 = d2.m;
+  a.p1 = d1.p;
+  a.p2 = d2.p;
+  const Tiling& tl = pl.mma;
+  a.rows_per_chunk = tl.chunk_rows();
+  a.ntx = tl.n_tiles1;
+  a.nty = tl.n_chunks2;
+  a.gmax = tl.w1max * tl.w2max;
+  a.src.ncp = 1;
+  a.nsamp = 1;
+  for (int d = 0; d < pl.nin; ++d) {
+    a.src.ncp *= pl.dims[d].n;
+    a.nsamp *= pl.dims[d].m;
+  }
+  if (pl.nin == 3) {
+    a.src.span3 = pl.dims[2].span;
+    a.src.p3 = pl.dims[2].p;
+    a.src.tab3 = pl.dims[2].ktab32;
+  }
+  a.tab1[0] = d1.ktab32;
+  a.tab2[0] = d2.ktab32;
+  a.s1[0] = ldexpf(1.f, -d1.tab_exp[0]);
+  a.s2[0] = ldexpf(1.f, -d2.tab_exp[0]);
+  a.e12[0] = d1.tab_exp[0] + d2.tab_exp[0];
+  a.out[0] = out;
+  a.nin = pl.nin;
+  a.nc = nc;
+  a.nurbs = pl.nurbs;
+  std::memcpy(record, &a, sizeof(a));
+}
+
+static const void* batch_mma_kernel(int tier) {
+  return tier == SG_TIER_SPLIT_F16 ? reinterpret_cast<const void*>(batch_mma<SG_TIER_SPLIT_F16>)
+                                   : reinterpret_cast<const void*>(batch_mma<SG_TIER_BF16>);
+}
+
+int batch_mma_resident(int tier, size_t lds) {
+  int blocks = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, batch_mma_kernel(tier), kThreads,
+                                                   lds) != hipSuccess)
+    return 0;
+  return blocks;
+}
+
+hipError_t launch_batch_mma(int tier, const void* args, const void* tiles, int* counters,
+                            int blocks, size_t lds, hipStream_t s) {
+  hipLaunchKernelGGL(queue_reset, dim3(1), dim3(64), 0, s, counters);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const auto* a = static_cast<const MmaArgs*>(args);
+  const auto* t = static_cast<const BTile*>(tiles);
+  if (tier == SG_TIER_SPLIT_F16)
+    hipLaunchKernelGGL(batch_mma<SG_TIER_SPLIT_F16>, dim3(blocks), dim3(kThreads), lds, s, a, t,
+                       counters);
+  else
+    hipLaunchKernelGGL(batch_mma<SG_TIER_BF16>, dim3(blocks), dim3(kThreads), lds, s, a, t,
+                       counters);
+  return hipGetLastError();
 }
 
 }  // namespace sg

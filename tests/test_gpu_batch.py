@@ -114,7 +114,39 @@ def test_batch_rejects_bad_input(gpu):
                                 rng, dtype=np.float64)
     with pytest.raises(TypeError, match="FP32"):
         sg.SplineGridBatch([g64])
+    # the edge-case grids (degree 7, unsorted samples) cannot take the MFMA tiers
     grids = _batch_grids(sg, rng, n=4)
     batch = sg.SplineGridBatch(grids)
-    with pytest.raises(sg.SplineGridsError, match="native tier only"):
+    assert "mfma_tiers=0" in batch.describe()
+    with pytest.raises(sg.SplineGridsError, match="every grid of the batch"):
         batch.evaluate_(tier="split_f16")
+
+
+@pytest.mark.parametrize("tier", ["split_f16", "bf16"])
+def test_batch_mfma_tiers(gpu, tier):
+    """The split-FP16 and BF16 tiers over a batch: the persistent launch gives
+    every grid exactly its own tier launch's result (same tile code), and the
+    oracle bound of the tier."""
+    sg = gpu
+    rng = np.random.default_rng(25)
+    specs = bench.c5_specs(rng, 40, small=True) + [
+        dict(n=(9, 12, 7), p=(1, 2, 0), m=(20, 17, 7), nout=3, nurbs=True),
+        dict(n=(8, 10, 12), p=(4, 1, 5), m=(30, 22, 13), nout=5, nurbs=True),
+        dict(n=(10, 8, 6), p=(2, 3, 1), m=(33, 20, 9), nout=6, nurbs=False),
+        dict(n=(9, 7), p=(0, 5), m=(70, 50), nout=2, nurbs=False),
+    ]
+    grids = [bench.build_spec_grid(sg, spec, rng) for spec in specs]
+    batch = sg.SplineGridBatch(grids)
+    assert "mfma_tiers=1" in batch.describe(), batch.describe()
+    batch.evaluate_(tier=tier)
+    torch.cuda.synchronize()
+    got = [g.eval.clone() for g in grids]
+    for g in grids:
+        g.eval.zero_()
+        sg.evaluate_(g, tier=tier)
+    torch.cuda.synchronize()
+    for i, (g, r) in enumerate(zip(grids, got)):
+        assert torch.equal(g.eval, r), i
+    bar = 1e-6 if tier == "split_f16" else 2e-2
+    for i, g in enumerate(grids):
+        assert _check_against_oracle(sg, g) < bar, i
