@@ -165,9 +165,12 @@ int sg_batch_destroy(sg_batch* batch);
 int sg_batch_set_arrays(sg_batch* batch, const void* const* control_points,
                         const void* const* weights, void* const* evals);
 
-/* Enqueue the evaluation of every grid: a one-thread kernel that resets the
- * work-queue counter, then the persistent kernel.  Only kernel launches, so
- * the call can be captured into a hipGraph and replayed. */
+/* Enqueue the evaluation of every grid: a small kernel that resets the
+ * work-queue counters, then the persistent kernel.  Only kernel launches, so
+ * the call can be captured into a hipGraph and replayed.  tier: native, or
+ * split-FP16 / BF16 when every grid of the batch takes the MFMA tiers
+ * (SG_ERR_UNSUPPORTED otherwise); each grid then gets the result of its own
+ * sg_evaluate with that tier. */
 int sg_batch_evaluate(sg_batch* batch, int32_t tier, void* stream);
 
 /* The loop that a batch replaces, for comparison: one sg_evaluate-style
