@@ -132,6 +132,24 @@ int sg_evaluate_fields(sg_plan* plan, int32_t n_fields,
                        const void* control_points, void* const* evals,
                        int32_t dtype, int32_t tier, void* stream);
 
+/* Derivative fields of a NURBS plan's rational map R = sum N w P / sum N w.
+ * No counterpart in the reference, which rejects NURBS derivatives
+ * (src/validation.jl:81-88; sg_evaluate keeps that behaviour): the quotient
+ * rule over the homogeneous B-spline fields of (w P, w), combined in FP64.
+ * derivative_orders is [n_fields][n_in] (n_fields <= SG_MAX_FIELDS, each
+ * order <= the dimension's max_derivative_order, at most 16 orders below
+ * the requested ones); evals[f] has the shape of eval.  workspace: device
+ * memory of sg_rational_workspace_bytes bytes (homogeneous control points
+ * and fields); the call allocates nothing. */
+int sg_rational_workspace_bytes(const sg_plan* plan, int32_t n_fields,
+                                const int32_t* derivative_orders, int32_t dtype,
+                                int64_t* bytes);
+int sg_evaluate_rational_fields(sg_plan* plan, int32_t n_fields,
+                                const int32_t* derivative_orders,
+                                const void* control_points, const void* weights,
+                                void* const* evals, void* workspace,
+                                int64_t workspace_bytes, int32_t dtype, void* stream);
+
 /* evaluate_adjoint!(grid; derivative_order, control_points, eval): replaces
  * src/adjoint.jl:52-83 (kernel :1-40).  Computes M^T eval into
  * control_points without atomics; the result is deterministic.

@@ -28,6 +28,10 @@ NURBS branch               src/spline_grid.jl:142-182          G4 unit circle (t
                                                                ratio of two B-spline evaluations (w*P and w)
 adjoint_direct             src/adjoint.jl:1-83                 G2 index set (test/test_local_refinement.jl:
                                                                48-67), <Mx, y> = <x, M^T y>, dense M^T
+rational_derivs            none: the reference rejects NURBS   unit weights = B-spline derivatives, the NURBS
+                           derivatives (src/validation.jl:     circle's R . dR/dt = 0, central finite
+                           81-88); The NURBS Book eq. 4.8 /    differences of eval_direct with weights
+                           Alg. A4.4 (Piegl & Tiller)
 =========================  ==================================  ================================================
 
 Every function above has at least one such check, so no oracle function is
@@ -288,3 +292,42 @@ def adjoint_direct(dims: Sequence[Dim], ebar: np.ndarray, deriv=None,
     else:
         lib().sgo_adjoint_direct_f32(*args, ctypes.c_int32(nthreads))
     return out
+
+
+def _binom(n: int, k: int) -> int:
+    from math import comb
+    return comb(n, k)
+
+
+def rational_derivs(dims: Sequence[Dim], P: np.ndarray, W: np.ndarray, deriv,
+                    nthreads: int = 1) -> np.ndarray:
+    """Partial derivative ``deriv`` of the rational map R = (sum N w P) / (sum N w)
+    of a NURBS grid, in FP64 (Piegl & Tiller, The NURBS Book, eq. 4.8 and
+    Alg. A4.4, for any number of dimensions): with A = sum N w P and
+    w(x) = sum N w, Leibniz's rule on A = w R gives
+        R^(a) = (A^(a) - sum_{0 < b <= a} C(a, b) w^(b) R^(a-b)) / w,
+    C(a, b) = prod_d binom(a_d, b_d).  A^(b) and w^(b) are B-spline derivative
+    fields of (w P) and w (eval_direct).  The reference itself rejects NURBS
+    derivatives (src/validation.jl:81-88)."""
+    import itertools
+
+    deriv = tuple(int(x) for x in deriv)
+    nin = len(dims)
+    Wf = np.asarray(W, dtype=np.float64)
+    H = np.asarray(P, dtype=np.float64) * Wf[..., None]
+    lower = sorted(itertools.product(*[range(a + 1) for a in deriv]), key=lambda b: (sum(b), b))
+    A = {b: eval_direct(dims, H, deriv=b, nthreads=nthreads) for b in lower}
+    w = {b: eval_direct(dims, Wf[..., None], deriv=b, nthreads=nthreads)[..., 0] for b in lower}
+    R = {}
+    for g in lower:
+        acc = A[g].copy()
+        for b in lower:
+            if sum(b) == 0 or any(b[d] > g[d] for d in range(nin)):
+                continue
+            c = 1
+            for d in range(nin):
+                c *= _binom(g[d], b[d])
+            rest = tuple(g[d] - b[d] for d in range(nin))
+            acc -= c * w[b][..., None] * R[rest]
+        R[g] = acc / w[(0,) * nin][..., None]
+    return R[deriv]

@@ -24,6 +24,7 @@ EXPORTS = (
     "sg_evaluate_fields", "sg_evaluate_adjoint", "sg_stream_synchronize",
     "sg_batch_create", "sg_batch_destroy", "sg_batch_set_arrays", "sg_batch_evaluate",
     "sg_batch_evaluate_each", "sg_batch_autotune", "sg_batch_info", "sg_batch_describe",
+    "sg_rational_workspace_bytes", "sg_evaluate_rational_fields",
 )
 
 
@@ -92,6 +93,9 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
         "sg_batch_info": (ctypes.c_int, [vp, ctypes.POINTER(i64), ctypes.POINTER(i64), i32p,
                                          i32p, i32p]),
         "sg_batch_describe": (ctypes.c_int, [vp, ctypes.c_char_p, i32]),
+        "sg_rational_workspace_bytes": (ctypes.c_int, [vp, i32, i32p, i32, ctypes.POINTER(i64)]),
+        "sg_evaluate_rational_fields": (ctypes.c_int, [vp, i32, i32p, vp, vp, ctypes.POINTER(vp),
+                                                       vp, i64, i32, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

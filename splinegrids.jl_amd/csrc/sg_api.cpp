@@ -715,6 +715,66 @@ int sg_evaluate_fields(sg_plan* plan, int32_t n_fields, const int32_t* derivativ
                  "forward launch");
 }
 
+// Orders of rational derivative fields: 0 <= o <= D per dimension.
+static int check_rational(const sg_plan* plan, int32_t n_fields, const int32_t* orders,
+                          int32_t dtype) {
+  if (!plan) return fail(SG_ERR_INVALID, "plan is NULL");
+  if (dtype != SG_F32 && dtype != SG_F64) return fail(SG_ERR_INVALID, "dtype must be SG_F32 or SG_F64");
+  const Plan& pl = plan->p;
+  if (!pl.nurbs) return fail(SG_ERR_INVALID, "rational derivatives need a NURBS plan");
+  if (n_fields < 1 || n_fields > SG_MAX_FIELDS)
+    return fail(SG_ERR_INVALID, "n_fields must be 1.." + std::to_string(SG_MAX_FIELDS));
+  if (!orders) return fail(SG_ERR_INVALID, "derivative_orders is NULL");
+  for (int f = 0; f < n_fields; ++f)
+    for (int d = 0; d < pl.nin; ++d) {
+      const int o = orders[f * pl.nin + d];
+      if (o < 0 || o > pl.dims[d].D) {
+        std::ostringstream os;
+        os << "The maximum derivative order available for spline dimension " << d + 1
+           << " is " << pl.dims[d].D << ", got " << o << ".";
+        return fail(SG_ERR_INVALID, os.str());
+      }
+    }
+  if (!rational_closure_ok(pl, n_fields, orders))
+    return fail(SG_ERR_UNSUPPORTED, "too many derivative orders below the requested ones "
+                                    "(at most 16)");
+  return SG_OK;
+}
+
+int sg_rational_workspace_bytes(const sg_plan* plan, int32_t n_fields,
+                                const int32_t* derivative_orders, int32_t dtype,
+                                int64_t* bytes) {
+  int rc = check_rational(plan, n_fields, derivative_orders, dtype);
+  if (rc != SG_OK) return rc;
+  if (!bytes) return fail(SG_ERR_INVALID, "bytes is NULL");
+  *bytes = static_cast<int64_t>(rational_workspace_bytes(
+      plan->p, n_fields, derivative_orders, dtype == SG_F32 ? sizeof(float) : sizeof(double)));
+  return SG_OK;
+}
+
+int sg_evaluate_rational_fields(sg_plan* plan, int32_t n_fields, const int32_t* derivative_orders,
+                                const void* control_points, const void* weights,
+                                void* const* evals, void* workspace, int64_t workspace_bytes,
+                                int32_t dtype, void* stream) {
+  int rc = check_rational(plan, n_fields, derivative_orders, dtype);
+  if (rc != SG_OK) return rc;
+  const Plan& pl = plan->p;
+  if (n_samples(pl) == 0) return SG_OK;
+  if (!control_points || !weights || !evals || !workspace)
+    return fail(SG_ERR_INVALID, "null device pointer");
+  for (int f = 0; f < n_fields; ++f)
+    if (!evals[f]) return fail(SG_ERR_INVALID, "null device pointer");
+  const size_t need = rational_workspace_bytes(pl, n_fields, derivative_orders,
+                                               dtype == SG_F32 ? sizeof(float) : sizeof(double));
+  if (workspace_bytes < 0 || static_cast<size_t>(workspace_bytes) < need)
+    return fail(SG_ERR_INVALID, "workspace too small: " + std::to_string(need) + " bytes needed");
+  DeviceGuard guard(pl.device);
+  if (guard.err != hipSuccess) return hip_fail(guard.err, "hipSetDevice");
+  return checked(launch_rational(pl, n_fields, derivative_orders, control_points, weights, evals,
+                                 workspace, dtype, static_cast<hipStream_t>(stream)),
+                 "rational launch");
+}
+
 int sg_evaluate_adjoint(sg_plan* plan, const int32_t* derivative_order, const void* eval,
                         void* control_points, int32_t dtype, int32_t tier, int32_t accumulate,
                         void* stream) {

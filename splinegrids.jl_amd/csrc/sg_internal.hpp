@@ -226,6 +226,16 @@ int batch_mma_resident(int tier, size_t lds);
 hipError_t launch_batch_mma(int tier, const void* args, const void* tiles, int* counters,
                             int blocks, size_t lds, hipStream_t s);
 
+// ---- sg_rational.hip ----
+// Derivative fields of a NURBS plan's rational map (quotient rule over the
+// homogeneous B-spline fields); the workspace holds the homogeneous control
+// points and fields.
+int rational_closure_ok(const Plan& plan, int nf, const int32_t* orders);
+size_t rational_workspace_bytes(const Plan& plan, int nf, const int32_t* orders, size_t elem);
+hipError_t launch_rational(const Plan& plan, int nf, const int32_t* orders, const void* P,
+                           const void* W, void* const* outs, void* work, int dtype,
+                           hipStream_t s);
+
 // ---- sg_adjoint.hip ----
 hipError_t launch_adjoint(const Plan& plan, const int32_t* orders,
                           const void* ebar, void* Pbar, int dtype,

@@ -235,6 +235,44 @@ def evaluate_fields_(grid: SplineGrid, derivative_orders, evals, *,
                                         _lib.TIERS[tier], current_stream_handle(grid.device)))
 
 
+def evaluate_rational_fields_(grid: SplineGrid, derivative_orders, evals, *,
+                              control_points=None) -> None:
+    """Derivative fields of a NURBS grid's rational map R = sum N w P / sum N w.
+
+    The reference rejects NURBS derivatives (src/validation.jl:81-88), and so
+    does evaluate_; this extension applies the quotient rule to the
+    homogeneous B-spline fields of (w P, w) and combines them in FP64.  The
+    device workspace is allocated on the first call and kept on the grid."""
+    assert grid.is_nurbs, "rational derivatives need a NURBS grid"
+    orders = [tuple(int(o) for o in d) for d in derivative_orders]
+    assert len(orders) == len(evals) and 1 <= len(orders) <= _lib.MAX_FIELDS
+    for o in orders:
+        assert len(o) == grid.Nin, "one derivative order per input dimension"
+    P = obtain(grid.control_points if control_points is None else control_points)
+    _check_array("control_points", P, grid)
+    _check_array("weights", grid.weights, grid)
+    for ev in evals:
+        assert tuple(ev.shape) == tuple(grid.eval.shape)
+        _check_array("eval", ev, grid)
+    plan = grid.plan()
+    L = _lib.lib()
+    flat = _lib.i32_array([o for d in orders for o in d])
+    dt = dtype_code(grid.dtype)
+    nbytes = ctypes.c_int64(0)
+    _lib.check(L.sg_rational_workspace_bytes(plan.handle, len(orders), flat, dt,
+                                             ctypes.byref(nbytes)))
+    ws = getattr(grid, "_rational_workspace", None)
+    if ws is None or ws.numel() < nbytes.value:
+        ws = torch.empty(max(nbytes.value, 1), dtype=torch.uint8, device=grid.device)
+        grid._rational_workspace = ws
+    ptrs = (ctypes.c_void_p * len(evals))(*[e.data_ptr() for e in evals])
+    with torch.cuda.device(grid.device):
+        _lib.check(L.sg_evaluate_rational_fields(plan.handle, len(orders), flat, P.data_ptr(),
+                                                 grid.weights.data_ptr(), ptrs, ws.data_ptr(),
+                                                 ws.numel(), dt,
+                                                 current_stream_handle(grid.device)))
+
+
 def evaluate_adjoint_(grid: SplineGrid, *, derivative_order=None, control_points=None,
                       eval=None, tier: str = "native", accumulate: bool = False,
                       synchronize: bool = False) -> None:

@@ -297,3 +297,69 @@ def test_planted_mistakes_are_caught(scipy_vectors, goldens):
     Pbar = O.adjoint_direct(dims2, err)
     Pbar_bad = np.roll(Pbar, 1, axis=0)          # shifted band ownership
     assert g2_indices(Pbar_bad) != goldens["G2"]["refinement_indices_1based"]
+
+
+# ---- rational derivatives (NURBS): pinned by the mathematics ----
+
+def test_rational_derivs_unit_weights_are_bspline_derivs():
+    rng = np.random.default_rng(31)
+    dims = [O.Dim.default(7, 3, 13, 2), O.Dim.default(6, 2, 11, 2)]
+    P = rng.standard_normal((7, 6, 2))
+    W = np.ones((7, 6))
+    for deriv in [(1, 0), (0, 1), (1, 1), (2, 1)]:
+        got = O.rational_derivs(dims, P, W, deriv)
+        want = O.eval_direct(dims, P, deriv=deriv)
+        assert _rel(got, want) < 1e-12, deriv
+
+
+def test_rational_derivs_circle_tangent(goldens):
+    """|R| = 1 on the NURBS circle, so R . dR/dt = 0 and dR/dt != 0."""
+    g = goldens["G4"]
+    knots = O.expand_knots(np.array(g["knot_values"]), np.array(g["multiplicities"]))
+    t = O.sample_points((knots[0], knots[-1]), g["n_sample_points"], np.float64)
+    d = O.Dim(knots, t, g["degree"], 1)
+    P = np.array(g["control_points"])
+    W = np.ones(g["n_control_points"])
+    W[1::2] = g["weights_even_1based"]
+    R = O.eval_direct([d], P, W=W)
+    dR = O.rational_derivs([d], P, W, (1,))
+    speed = np.linalg.norm(dR, axis=1)
+    assert speed.min() > 0.5
+    np.testing.assert_allclose(np.sum(R * dR, axis=1) / speed, 0.0, atol=1e-12)
+
+
+def test_rational_derivs_finite_differences():
+    """Central differences of the NURBS map in each sample coordinate (no
+    sample on an interior knot, where a higher derivative jumps)."""
+    rng = np.random.default_rng(32)
+    n, p, m = (6, 5, 4), (3, 2, 2), (9, 8, 8)
+    dims = [O.Dim.default(n[i], p[i], m[i], 2) for i in range(3)]
+    P = rng.standard_normal(n + (2,))
+    W = rng.uniform(0.5, 2.0, n)
+    h = 1e-5
+
+    def shifted(axis, dt):
+        out = []
+        for i, d in enumerate(dims):
+            t = d.t + (dt if i == axis else 0.0)
+            t = np.clip(t, d.knots_all[0], d.knots_all[-1])
+            out.append(O.Dim(d.knots_all, t, d.degree, d.max_deriv))
+        return out
+
+    def interior(axis):
+        sl = [slice(None)] * 3
+        sl[axis] = slice(1, -1)  # the shift clips at the ends of the domain
+        return tuple(sl)
+
+    for axis in range(3):
+        e = [0, 0, 0]
+        e[axis] = 1
+        got = O.rational_derivs(dims, P, W, tuple(e))
+        fd = (O.eval_direct(shifted(axis, h), P, W=W)
+              - O.eval_direct(shifted(axis, -h), P, W=W)) / (2 * h)
+        assert _rel(got[interior(axis)], fd[interior(axis)]) < 1e-6, axis
+    # a mixed partial: differences of the first derivative along dimension 2
+    got = O.rational_derivs(dims, P, W, (1, 1, 0))
+    fd = (O.rational_derivs(shifted(1, h), P, W, (1, 0, 0))
+          - O.rational_derivs(shifted(1, -h), P, W, (1, 0, 0))) / (2 * h)
+    assert _rel(got[interior(1)], fd[interior(1)]) < 1e-6
