@@ -7,8 +7,8 @@ knots, control points N(0, 1) in FP32, derivative fields (0,0), (1,0),
 (0,1), (1,1) evaluated in one fused launch.  value = sample-points/s =
 fields * samples / time.  Inputs are resident in HBM before timing starts.
 
-Prints one JSON line.  Other workloads (--workload c1|c3|c3n|c4|c4a) are
-for profiling and parity runs.
+Prints one JSON line.  Other workloads (--workload c1|c3|c3n|c3d|c4|c4a|c5)
+are for profiling and parity runs.
 """
 from __future__ import annotations
 
@@ -73,6 +73,15 @@ def workload(name: str):
                     n=(64, 64, 32), p=(3, 3, 2), m=(512, 512, 256), D=(0, 0, 0), nout=3,
                     dtype=np.float32, knots=None, fields=[(0, 0, 0)], P=P,
                     W=W if name == "c3" else None)
+    if name == "c3d":  # gradient of the NURBS map (quotient rule, an extension)
+        rng = np.random.default_rng(SEED0 + 3)
+        P = rng.uniform(-1, 1, (64, 64, 32, 3)).astype(np.float32)
+        W = rng.uniform(0.5, 2.0, (64, 64, 32)).astype(np.float32)
+        return dict(name="c3d: gradient of the 3-D->3 NURBS map, n=(64,64,32), p=(3,3,2), "
+                         "m=(512,512,256), fields (1,0,0), (0,1,0), (0,0,1), FP32",
+                    n=(64, 64, 32), p=(3, 3, 2), m=(512, 512, 256), D=(1, 1, 1), nout=3,
+                    dtype=np.float32, knots=None,
+                    fields=[(1, 0, 0), (0, 1, 0), (0, 0, 1)], P=P, W=W, rational=True)
     if name in ("copy", "fill"):  # calibration: plain device copy / fill of 1 GiB
         what = "device-to-device copy" if name == "copy" else "fill (write only)"
         return dict(name=f"{name}: 1 GiB {what} (bandwidth and counter calibration)",
@@ -213,7 +222,10 @@ def fp64_errors(sg, grid, wl, outs):
     errs, rms = [], []
     ref = sg.colmajor_empty(g64.eval.shape, torch.float64, "cuda")
     for f, out in zip(wl["fields"], outs):
-        sg.evaluate_(g64, derivative_order=f, eval=ref)
+        if wl.get("rational"):
+            sg.evaluate_rational_fields_(g64, [f], [ref])
+        else:
+            sg.evaluate_(g64, derivative_order=f, eval=ref)
         d = out.double() - ref
         scale = float(ref.abs().max()) or 1.0  # an all-zero field: absolute error
         errs.append(float(d.abs().max()) / scale)
@@ -467,7 +479,9 @@ def main(argv=None):
 
     def make_step(tier):
         def step():
-            if adjoint:
+            if wl.get("rational"):
+                sg.evaluate_rational_fields_(grid, fields, outs)
+            elif adjoint:
                 sg.evaluate_adjoint_(grid, eval=ebar, control_points=pbar, tier=tier)
             elif len(fields) == 1:
                 sg.evaluate_(grid, derivative_order=fields[0], tier=tier)
@@ -514,7 +528,8 @@ def main(argv=None):
         out["max_rel_err_vs_fp64"], out["rms_rel_err_vs_fp64"] = fp64_adjoint_errors(
             sg, grid, ebar, pbar)
     # the other arithmetic tiers on the same inputs, timed the same way
-    if wl["dtype"] == np.float32 and args.tier == "native" and not args.no_tiers:
+    if (wl["dtype"] == np.float32 and args.tier == "native" and not args.no_tiers
+            and not wl.get("rational")):
         tiers = {"native": {"ms_per_step": ms, "value": value,
                             "max_rel_err_vs_fp64": out["max_rel_err_vs_fp64"],
                             "rms_rel_err_vs_fp64": out["rms_rel_err_vs_fp64"]}}
@@ -536,7 +551,8 @@ def main(argv=None):
             sg.evaluate_adjoint_(grid, eval=ebar, control_points=pbar)
         out["forward_plus_adjoint_ms"], _ = timed_steps(fwd_bwd, max(20, args.steps // 2),
                                                         args.warmup)
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and not wl.get("rational"):
+        # (the reference has no NURBS derivatives to time)
         out["cpu_baseline"] = cpu_baseline(wl, args.cpu_budget)
         out["cpu_baseline"]["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
     print(json.dumps(out), flush=True)

@@ -45,26 +45,37 @@ __global__ void homogenize(const T* P, const T* W, T* H, long long ncp, int nout
   H[i + ncp * nout] = w;
 }
 
+// One thread per sample, all channels in turn (the weight fields are read
+// once).  A thread's closure values sit in a column of LDS, [2][K][256]
+// doubles (w, then R), indexed at run time without scratch memory.
+constexpr int kCombineThreads = 256;
+
 template <typename T>
-__global__ void rational_combine(RatArgs<T> a) {
-  const long long gid = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
-  if (gid >= a.nsamp * a.nout) return;
-  const int c = static_cast<int>(gid / a.nsamp);
-  const long long j = gid - static_cast<long long>(c) * a.nsamp;
+__global__ __launch_bounds__(kCombineThreads) void rational_combine(RatArgs<T> a) {
+  extern __shared__ double sh[];
+  constexpr int S = kCombineThreads;  // stride between a thread's entries
+  const int K = a.nclosure;
+  double* w = sh + threadIdx.x;
+  double* R = sh + static_cast<size_t>(K) * S + threadIdx.x;
   const long long fstride = a.nsamp * (a.nout + 1);
-  double R[kMaxClosure], w[kMaxClosure];
-  for (int g = 0; g < a.nclosure; ++g) {
-    R[g] = static_cast<double>(a.F[g * fstride + c * a.nsamp + j]);
-    w[g] = static_cast<double>(a.F[g * fstride + a.nout * a.nsamp + j]);
+  for (long long j = static_cast<long long>(blockIdx.x) * S + threadIdx.x; j < a.nsamp;
+       j += static_cast<long long>(gridDim.x) * S) {
+    for (int g = 0; g < K; ++g)
+      w[g * S] = static_cast<double>(a.F[g * fstride + a.nout * a.nsamp + j]);
+    for (int c = 0; c < a.nout; ++c) {
+      for (int g = 0; g < K; ++g)
+        R[g * S] = static_cast<double>(a.F[g * fstride + c * a.nsamp + j]);
+      int t = 0;
+      for (int g = 0; g < K; ++g) {
+        double acc = R[g * S];
+        for (; t < a.nterms && a.tg[t] == g; ++t)
+          acc -= static_cast<double>(a.coef[t]) * w[a.tb[t] * S] * R[a.tr[t] * S];
+        R[g * S] = acc / w[0];
+      }
+      for (int f = 0; f < a.nfields; ++f)
+        a.out[f][c * a.nsamp + j] = static_cast<T>(R[a.out_idx[f] * S]);
+    }
   }
-  int t = 0;
-  for (int g = 0; g < a.nclosure; ++g) {
-    double acc = R[g];
-    for (; t < a.nterms && a.tg[t] == g; ++t)
-      acc -= static_cast<double>(a.coef[t]) * w[a.tb[t]] * R[a.tr[t]];
-    R[g] = acc / w[0];
-  }
-  for (int f = 0; f < a.nfields; ++f) a.out[f][c * a.nsamp + j] = static_cast<T>(R[a.out_idx[f]]);
 }
 
 using Order = std::array<int, SG_MAX_NIN>;
@@ -161,9 +172,14 @@ hipError_t rational(const Plan& pl, int nf, const int32_t* orders, const T* P, c
   if (e != hipSuccess) return e;
   // pass 2: the quotient rule
   a.F = F;
-  const long long threads = nsamp * pl.nout;
-  hipLaunchKernelGGL(rational_combine<T>, dim3(static_cast<unsigned>((threads + 255) / 256)),
-                     dim3(256), 0, s, a);
+  int dev = 0, cus = 256;
+  if (hipGetDevice(&dev) == hipSuccess)
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  const long long blocks = std::min<long long>((nsamp + kCombineThreads - 1) / kCombineThreads,
+                                               static_cast<long long>(cus) * 8);
+  const size_t lds = 2 * static_cast<size_t>(K) * kCombineThreads * sizeof(double);
+  hipLaunchKernelGGL(rational_combine<T>, dim3(static_cast<unsigned>(std::max(1LL, blocks))),
+                     dim3(kCombineThreads), lds, s, a);
   return hipGetLastError();
 }
 
