@@ -150,6 +150,18 @@ int sg_evaluate_rational_fields(sg_plan* plan, int32_t n_fields,
                                 void* const* evals, void* workspace,
                                 int64_t workspace_bytes, int32_t dtype, void* stream);
 
+/* The adjoint of a NURBS plan's evaluate! (derivative order 0) in its
+ * control points, the weights fixed: control_points = w_k [B^T (eval /
+ * w(x))]_k.  No counterpart in the reference, whose evaluate_adjoint! rejects
+ * NURBS (src/adjoint.jl:57; sg_evaluate_adjoint keeps that behaviour).
+ * accumulate as in sg_evaluate_adjoint.  workspace: device memory of
+ * sg_rational_adjoint_workspace_bytes bytes; the call allocates nothing. */
+int sg_rational_adjoint_workspace_bytes(const sg_plan* plan, int32_t dtype, int64_t* bytes);
+int sg_evaluate_rational_adjoint(sg_plan* plan, const void* weights, const void* eval,
+                                 void* control_points, void* workspace,
+                                 int64_t workspace_bytes, int32_t dtype, int32_t accumulate,
+                                 void* stream);
+
 /* evaluate_adjoint!(grid; derivative_order, control_points, eval): replaces
  * src/adjoint.jl:52-83 (kernel :1-40).  Computes M^T eval into
  * control_points without atomics; the result is deterministic.

@@ -28,6 +28,9 @@ NURBS branch               src/spline_grid.jl:142-182          G4 unit circle (t
                                                                ratio of two B-spline evaluations (w*P and w)
 adjoint_direct             src/adjoint.jl:1-83                 G2 index set (test/test_local_refinement.jl:
                                                                48-67), <Mx, y> = <x, M^T y>, dense M^T
+adjoint_rational           none: the reference rejects the     the transpose identity <R(P), e> = <P, R^T e>
+                           NURBS adjoint (src/adjoint.jl:57)   against eval_direct with weights; unit weights
+                                                               = adjoint_direct
 rational_derivs            none: the reference rejects NURBS   unit weights = B-spline derivatives, the NURBS
                            derivatives (src/validation.jl:     circle's R . dR/dt = 0, central finite
                            81-88); The NURBS Book eq. 4.8 /    differences of eval_direct with weights
@@ -331,3 +334,14 @@ def rational_derivs(dims: Sequence[Dim], P: np.ndarray, W: np.ndarray, deriv,
             acc -= c * w[b][..., None] * R[rest]
         R[g] = acc / w[(0,) * nin][..., None]
     return R[deriv]
+
+
+def adjoint_rational(dims: Sequence[Dim], W: np.ndarray, ebar: np.ndarray,
+                     nthreads: int = 1) -> np.ndarray:
+    """Adjoint of the NURBS map in its control points, weights fixed, FP64:
+    R = B (w P) / w(x) is linear in P, so R^T e = w * B^T (e / w(x)).  The
+    reference's evaluate_adjoint! rejects NURBS (src/adjoint.jl:57)."""
+    Wf = np.asarray(W, dtype=np.float64)
+    wx = eval_direct(dims, Wf[..., None], nthreads=nthreads)[..., 0]
+    q = adjoint_direct(dims, np.asarray(ebar, dtype=np.float64) / wx[..., None])
+    return Wf[..., None] * q

@@ -11,8 +11,8 @@ from .knot_vector import KnotVector
 from .spline_dimension import SplineDimension, decompress
 from .control_points import DefaultControlPoints, get_n_control_points, obtain
 from .spline_grid import (NURBSGrid, Plan, SplineGrid, evaluate_, evaluate_adjoint_,
-                          evaluate_fields_, evaluate_rational_fields_, reshape_colmajor,
-                          vec_colmajor)
+                          evaluate_fields_, evaluate_rational_adjoint_,
+                          evaluate_rational_fields_, reshape_colmajor, vec_colmajor)
 from .validation import validate_partial_derivatives, validate_spline_grid
 from .autograd import SplineEval, spline_eval
 from .linear_map import LinearMap
@@ -25,7 +25,7 @@ __all__ = [
     "SplineGridsError", "lib", "KnotVector", "SplineDimension", "decompress",
     "DefaultControlPoints", "get_n_control_points", "obtain", "NURBSGrid", "Plan",
     "SplineGrid", "evaluate_", "evaluate_adjoint_", "evaluate_fields_",
-    "evaluate_rational_fields_",
+    "evaluate_rational_fields_", "evaluate_rational_adjoint_",
     "reshape_colmajor", "vec_colmajor", "validate_partial_derivatives",
     "validate_spline_grid", "colmajor", "colmajor_empty", "colmajor_zeros",
     "from_numpy_colmajor", "to_numpy_colmajor", "SplineEval", "spline_eval", "LinearMap",

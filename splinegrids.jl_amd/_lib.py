@@ -25,6 +25,7 @@ EXPORTS = (
     "sg_batch_create", "sg_batch_destroy", "sg_batch_set_arrays", "sg_batch_evaluate",
     "sg_batch_evaluate_each", "sg_batch_autotune", "sg_batch_info", "sg_batch_describe",
     "sg_rational_workspace_bytes", "sg_evaluate_rational_fields",
+    "sg_rational_adjoint_workspace_bytes", "sg_evaluate_rational_adjoint",
 )
 
 
@@ -96,6 +97,8 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
         "sg_rational_workspace_bytes": (ctypes.c_int, [vp, i32, i32p, i32, ctypes.POINTER(i64)]),
         "sg_evaluate_rational_fields": (ctypes.c_int, [vp, i32, i32p, vp, vp, ctypes.POINTER(vp),
                                                        vp, i64, i32, vp]),
+        "sg_rational_adjoint_workspace_bytes": (ctypes.c_int, [vp, i32, ctypes.POINTER(i64)]),
+        "sg_evaluate_rational_adjoint": (ctypes.c_int, [vp, vp, vp, vp, vp, i64, i32, i32, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

@@ -7,14 +7,16 @@ map control_points -> eval is linear, so the rules are exact:
   reverse:  dP = M^T d(eval)   (evaluate_adjoint_, accumulate into dP)
   forward:  d(eval) = M dP     (evaluate_ on the tangent)
 
-Basis tables and knots are constants.  B-spline grids only in reverse mode,
-as evaluate_adjoint! (src/adjoint.jl:57).
+Basis tables, knots and NURBS weights are constants.  In reverse mode a
+NURBS grid (derivative order 0) takes the rational adjoint,
+evaluate_rational_adjoint_ (the reference's evaluate_adjoint! rejects NURBS,
+src/adjoint.jl:57); it runs the native tier.
 """
 from __future__ import annotations
 
 import torch
 
-from .spline_grid import SplineGrid, evaluate_, evaluate_adjoint_
+from .spline_grid import SplineGrid, evaluate_, evaluate_adjoint_, evaluate_rational_adjoint_
 from .utils import colmajor, colmajor_empty, colmajor_zeros
 
 
@@ -33,10 +35,14 @@ class SplineEval(torch.autograd.Function):
     @staticmethod
     def backward(ctx, grad_eval: torch.Tensor):
         grid = ctx.grid
-        assert not grid.is_nurbs, "Adjoint evaluation not supported for NURBS."
         grad_cp = colmajor_zeros(tuple(grid.control_points.shape), grid.dtype, grid.device)
-        evaluate_adjoint_(grid, derivative_order=ctx.order, control_points=grad_cp,
-                          eval=colmajor(grad_eval), tier=ctx.tier, accumulate=True)
+        if grid.is_nurbs:
+            assert not any(ctx.order), "Computing derivatives of NURBS is currently not supported."
+            evaluate_rational_adjoint_(grid, control_points=grad_cp, eval=colmajor(grad_eval),
+                                       accumulate=True)
+        else:
+            evaluate_adjoint_(grid, derivative_order=ctx.order, control_points=grad_cp,
+                              eval=colmajor(grad_eval), tier=ctx.tier, accumulate=True)
         return grad_cp, None, None, None
 
     @staticmethod

@@ -478,7 +478,9 @@ int build_plan(const sg_grid_desc* desc, hipStream_t s, bool want_adjoint, Plan*
     pl->mma.xcd_map = xcd;
     ok = forward_lds_worst(*pl, pl->fwd) <= static_cast<size_t>(kMaxLds);
   }
-  if (ok && !pl->nurbs && want_adjoint) {
+  // adjoint tilings for NURBS plans too: the rational adjoint runs the
+  // B-spline adjoint on the same dimensions
+  if (ok && want_adjoint) {
     int rows = env_int("SG_ADJ_ROWS", 16);
     const int j3 = env_int("SG_ADJ_J3", 0);
     compute_tiling(*pl, pl->adj, rows);
@@ -494,7 +496,7 @@ int build_plan(const sg_grid_desc* desc, hipStream_t s, bool want_adjoint, Plan*
   }
   pl->stream_ok = ok;
   const char* adj_mma_env = std::getenv("SG_ADJ_MMA");  // "0": streaming kernel only
-  if (ok && !pl->nurbs && want_adjoint && !(adj_mma_env && std::strcmp(adj_mma_env, "0") == 0) &&
+  if (ok && want_adjoint && !(adj_mma_env && std::strcmp(adj_mma_env, "0") == 0) &&
       adjoint_mma_shape_ok(*pl))
     plan_adjoint_mma(*pl);
   if (ok && (pl->adj.n_tiles1 > 0 || pl->adjm.n_tiles1 > 0)) {
@@ -773,6 +775,35 @@ int sg_evaluate_rational_fields(sg_plan* plan, int32_t n_fields, const int32_t* 
   return checked(launch_rational(pl, n_fields, derivative_orders, control_points, weights, evals,
                                  workspace, dtype, static_cast<hipStream_t>(stream)),
                  "rational launch");
+}
+
+int sg_rational_adjoint_workspace_bytes(const sg_plan* plan, int32_t dtype, int64_t* bytes) {
+  if (!plan) return fail(SG_ERR_INVALID, "plan is NULL");
+  if (dtype != SG_F32 && dtype != SG_F64) return fail(SG_ERR_INVALID, "dtype must be SG_F32 or SG_F64");
+  if (!plan->p.nurbs) return fail(SG_ERR_INVALID, "the rational adjoint needs a NURBS plan");
+  if (!bytes) return fail(SG_ERR_INVALID, "bytes is NULL");
+  *bytes = static_cast<int64_t>(
+      rational_adjoint_workspace_bytes(plan->p, dtype == SG_F32 ? sizeof(float) : sizeof(double)));
+  return SG_OK;
+}
+
+int sg_evaluate_rational_adjoint(sg_plan* plan, const void* weights, const void* eval,
+                                 void* control_points, void* workspace, int64_t workspace_bytes,
+                                 int32_t dtype, int32_t accumulate, void* stream) {
+  int64_t need = 0;
+  int rc = sg_rational_adjoint_workspace_bytes(plan, dtype, &need);
+  if (rc != SG_OK) return rc;
+  const Plan& pl = plan->p;
+  if (n_control(pl) == 0) return SG_OK;
+  if (!weights || !control_points || !workspace || (!eval && n_samples(pl) > 0))
+    return fail(SG_ERR_INVALID, "null device pointer");
+  if (workspace_bytes < need)
+    return fail(SG_ERR_INVALID, "workspace too small: " + std::to_string(need) + " bytes needed");
+  DeviceGuard guard(pl.device);
+  if (guard.err != hipSuccess) return hip_fail(guard.err, "hipSetDevice");
+  return checked(launch_rational_adjoint(pl, weights, eval, control_points, workspace, dtype,
+                                         accumulate ? 1 : 0, static_cast<hipStream_t>(stream)),
+                 "rational adjoint launch");
 }
 
 int sg_evaluate_adjoint(sg_plan* plan, const int32_t* derivative_order, const void* eval,

@@ -235,6 +235,11 @@ size_t rational_workspace_bytes(const Plan& plan, int nf, const int32_t* orders,
 hipError_t launch_rational(const Plan& plan, int nf, const int32_t* orders, const void* P,
                            const void* W, void* const* outs, void* work, int dtype,
                            hipStream_t s);
+// The adjoint of a NURBS plan's evaluate! in its control points (weights
+// fixed): Pbar_k = w_k [B^T (ebar / w(x))]_k.
+size_t rational_adjoint_workspace_bytes(const Plan& plan, size_t elem);
+hipError_t launch_rational_adjoint(const Plan& plan, const void* W, const void* ebar, void* Pbar,
+                                   void* work, int dtype, int accumulate, hipStream_t s);
 
 // ---- sg_adjoint.hip ----
 hipError_t launch_adjoint(const Plan& plan, const int32_t* orders,

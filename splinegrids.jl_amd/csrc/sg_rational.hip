@@ -1,7 +1,11 @@
 // sg_rational.hip -- derivative fields of a NURBS grid's rational map
-// R = A / w, A = sum N w P, w = sum N w.  The reference rejects NURBS
-// derivatives (src/validation.jl:81-88); this is the quotient rule that its
-// fused numerator/denominator pass leaves room for.
+// R = A / w, A = sum N w P, w = sum N w, and its adjoint in the control
+// points.  The reference rejects both (src/validation.jl:81-88,
+// src/adjoint.jl:57).
+//
+// Adjoint: with the weights fixed R is linear in P, R = B (w P) / w(x), so
+// Pbar_k = w_k [B^T (ebar / w(x))]_k: the weight field by the B-spline
+// forward, a division, the B-spline adjoint, a product.
 //
 // Two passes.  (1) The homogeneous control points (w P_c, w) go through the
 // B-spline forward (launch_forward on a view of the plan without weights and
@@ -76,6 +80,63 @@ __global__ __launch_bounds__(kCombineThreads) void rational_combine(RatArgs<T> a
         a.out[f][c * a.nsamp + j] = static_cast<T>(R[a.out_idx[f] * S]);
     }
   }
+}
+
+// ebar / w(x), channel by channel (the weight field w(x) in D).
+template <typename T>
+__global__ void divide_by_field(const T* E, const T* D, T* out, long long nsamp, int nout) {
+  const long long i = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= nsamp) return;
+  const double inv = 1.0 / static_cast<double>(D[i]);
+  for (int c = 0; c < nout; ++c)
+    out[i + nsamp * c] = static_cast<T>(static_cast<double>(E[i + nsamp * c]) * inv);
+}
+
+// Pbar = (accumulate ? Pbar : 0) + w * Q, channel by channel.
+template <typename T>
+__global__ void weight_products(const T* Q, const T* W, T* Pbar, long long ncp, int nout,
+                                int accumulate) {
+  const long long i = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= ncp) return;
+  const T w = W[i];
+  for (int c = 0; c < nout; ++c) {
+    const T v = w * Q[i + ncp * c];
+    Pbar[i + ncp * c] = accumulate ? Pbar[i + ncp * c] + v : v;
+  }
+}
+
+template <typename T>
+hipError_t rational_adjoint(const Plan& pl, const T* W, const T* E, T* Pbar, void* work,
+                            int accumulate, hipStream_t s) {
+  long long ncp = 1, nsamp = 1;
+  for (const DimDev& d : pl.dims) {
+    ncp *= d.n;
+    nsamp *= d.m;
+  }
+  // workspace: w(x) [nsamp], ebar / w(x) [nsamp][nout], B^T (ebar / w) [ncp][nout]
+  T* D = static_cast<T*>(work);
+  T* Et = D + nsamp;
+  T* Q = Et + nsamp * pl.nout;
+  const int dtype = sizeof(T) == 4 ? SG_F32 : SG_F64;
+  const int32_t zero[SG_MAX_NIN] = {0, 0, 0, 0, 0, 0, 0, 0};
+  Plan wp = pl;  // a view: the weights as a one-channel B-spline
+  wp.nurbs = 0;
+  wp.nout = 1;
+  void* dout[1] = {D};
+  hipError_t e = launch_forward(wp, 1, zero, W, nullptr, dout, dtype, s);
+  if (e != hipSuccess) return e;
+  if (nsamp > 0) {
+    hipLaunchKernelGGL(divide_by_field<T>, dim3(static_cast<unsigned>((nsamp + 255) / 256)),
+                       dim3(256), 0, s, E, D, Et, nsamp, pl.nout);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
+  Plan bp = pl;  // a view: the B-spline adjoint of the same dimensions
+  bp.nurbs = 0;
+  e = launch_adjoint(bp, zero, Et, Q, dtype, 0, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(weight_products<T>, dim3(static_cast<unsigned>((ncp + 255) / 256)),
+                     dim3(256), 0, s, Q, W, Pbar, ncp, pl.nout, accumulate);
+  return hipGetLastError();
 }
 
 using Order = std::array<int, SG_MAX_NIN>;
@@ -219,6 +280,26 @@ hipError_t launch_rational(const Plan& pl, int nf, const int32_t* orders, const 
   return rational<double>(pl, nf, orders, static_cast<const double*>(P),
                           static_cast<const double*>(W), reinterpret_cast<double* const*>(outs),
                           work, s);
+}
+
+size_t rational_adjoint_workspace_bytes(const Plan& pl, size_t elem) {
+  long long ncp = 1, nsamp = 1;
+  for (const DimDev& d : pl.dims) {
+    ncp *= d.n;
+    nsamp *= d.m;
+  }
+  return elem * (static_cast<size_t>(nsamp) * (1 + pl.nout) + static_cast<size_t>(ncp) * pl.nout);
+}
+
+hipError_t launch_rational_adjoint(const Plan& pl, const void* W, const void* ebar, void* Pbar,
+                                   void* work, int dtype, int accumulate, hipStream_t s) {
+  if (dtype == SG_F32)
+    return rational_adjoint<float>(pl, static_cast<const float*>(W),
+                                   static_cast<const float*>(ebar), static_cast<float*>(Pbar),
+                                   work, accumulate, s);
+  return rational_adjoint<double>(pl, static_cast<const double*>(W),
+                                  static_cast<const double*>(ebar), static_cast<double*>(Pbar),
+                                  work, accumulate, s);
 }
 
 }  // namespace sg

@@ -273,6 +273,39 @@ def evaluate_rational_fields_(grid: SplineGrid, derivative_orders, evals, *,
                                                  current_stream_handle(grid.device)))
 
 
+def evaluate_rational_adjoint_(grid: SplineGrid, *, control_points=None, eval=None,
+                               accumulate: bool = False) -> None:
+    """The adjoint of a NURBS grid's evaluate_ (derivative order 0) in its
+    control points, the weights fixed: control_points = w_k [B^T (eval /
+    w(x))]_k, overwritten (or added to with ``accumulate=True``).
+
+    The reference's evaluate_adjoint! rejects NURBS (src/adjoint.jl:57), and
+    so does evaluate_adjoint_; this is the extension.  The device workspace is
+    allocated on the first call and kept on the grid."""
+    assert grid.is_nurbs, "the rational adjoint needs a NURBS grid"
+    P = obtain(grid.control_points if control_points is None else control_points)
+    ev = grid.eval if eval is None else eval
+    assert tuple(P.shape) == tuple(obtain(grid.control_points).shape)
+    assert tuple(ev.shape) == tuple(grid.eval.shape)
+    _check_array("control_points", P, grid)
+    _check_array("eval", ev, grid)
+    _check_array("weights", grid.weights, grid)
+    plan = grid.plan()
+    L = _lib.lib()
+    dt = dtype_code(grid.dtype)
+    nbytes = ctypes.c_int64(0)
+    _lib.check(L.sg_rational_adjoint_workspace_bytes(plan.handle, dt, ctypes.byref(nbytes)))
+    ws = getattr(grid, "_rational_adjoint_workspace", None)
+    if ws is None or ws.numel() < nbytes.value:
+        ws = torch.empty(max(nbytes.value, 1), dtype=torch.uint8, device=grid.device)
+        grid._rational_adjoint_workspace = ws
+    with torch.cuda.device(grid.device):
+        _lib.check(L.sg_evaluate_rational_adjoint(plan.handle, grid.weights.data_ptr(),
+                                                  ev.data_ptr(), P.data_ptr(), ws.data_ptr(),
+                                                  ws.numel(), dt, int(bool(accumulate)),
+                                                  current_stream_handle(grid.device)))
+
+
 def evaluate_adjoint_(grid: SplineGrid, *, derivative_order=None, control_points=None,
                       eval=None, tier: str = "native", accumulate: bool = False,
                       synchronize: bool = False) -> None:

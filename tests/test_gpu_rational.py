@@ -74,3 +74,53 @@ def test_rational_rejections(gpu):
     bs = sg.SplineGrid(tuple(make_dims(sg, (8, 7), (2, 3), (30, 25))), 1)
     with pytest.raises(AssertionError, match="NURBS grid"):
         sg.evaluate_rational_fields_(bs, [(0, 0)], [torch.empty_like(bs.eval)])
+
+
+# ---- the adjoint in the control points (weights fixed) ----
+
+ADJ_CASES = [
+    ((40, 40, 20), (3, 3, 3), (400, 96, 40), 3),   # matrix-core B-spline adjoint inside
+    ((60, 8, 5), (3, 2, 2), (64, 30, 9), 2),       # streaming adjoint inside
+    ((9, 8), (3, 2), (37, 29), 3),
+    ((12,), (3,), (60,), 2),                       # 1-D: the general adjoint inside
+]
+
+
+@pytest.mark.parametrize("dtype,bar", [(np.float64, 1e-12), (np.float32, 1e-5)])
+@pytest.mark.parametrize("case", ADJ_CASES, ids=lambda c: f"n{c[0]}")
+def test_rational_adjoint_vs_oracle(gpu, case, dtype, bar):
+    sg = gpu
+    n, p, m, nout = case
+    dims = make_dims(sg, n, p, m, dtype=dtype)
+    grid = sg.NURBSGrid(tuple(dims), nout)
+    rng = np.random.default_rng(sum(n) + nout)
+    W = rng.uniform(0.5, 2.0, n).astype(dtype)
+    E = rng.standard_normal(tuple(m) + (nout,)).astype(dtype)
+    grid.weights.copy_(sg.from_numpy_colmajor(W, grid.dtype, "cuda"))
+    ebar = sg.from_numpy_colmajor(E, grid.dtype, "cuda")
+    pbar = sg.colmajor_zeros(tuple(n) + (nout,), grid.dtype, "cuda")
+    sg.evaluate_rational_adjoint_(grid, eval=ebar, control_points=pbar)
+    want = O.adjoint_rational(oracle_dims(dims), W.astype(np.float64), E.astype(np.float64),
+                              nthreads=8)
+    assert rel_err(eval_host(sg, pbar), want) < bar
+    sg.evaluate_rational_adjoint_(grid, eval=ebar, control_points=pbar, accumulate=True)
+    assert rel_err(eval_host(sg, pbar), 2 * want) < bar
+
+
+def test_nurbs_reverse_mode(gpu):
+    """spline_eval on a NURBS grid differentiates through the rational
+    adjoint: d/dP sum(R(P) * E) = R^T E."""
+    sg = gpu
+    n, p, m = (8, 7, 5), (3, 2, 2), (30, 22, 13)
+    dims = make_dims(sg, n, p, m, dtype=np.float64)
+    grid = sg.NURBSGrid(tuple(dims), 2)
+    rng = np.random.default_rng(9)
+    W = rng.uniform(0.5, 2.0, n)
+    grid.weights.copy_(sg.from_numpy_colmajor(W, grid.dtype, "cuda"))
+    P = sg.from_numpy_colmajor(rng.standard_normal(n + (2,)), grid.dtype, "cuda")
+    P.requires_grad_(True)
+    E = rng.standard_normal(m + (2,))
+    out = sg.spline_eval(grid, P)
+    (out * sg.from_numpy_colmajor(E, grid.dtype, "cuda")).sum().backward()
+    want = O.adjoint_rational(oracle_dims(dims), W, E)
+    assert rel_err(sg.to_numpy_colmajor(P.grad), want) < 1e-12

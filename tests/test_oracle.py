@@ -363,3 +363,18 @@ def test_rational_derivs_finite_differences():
     fd = (O.rational_derivs(shifted(1, h), P, W, (1, 0, 0))
           - O.rational_derivs(shifted(1, -h), P, W, (1, 0, 0))) / (2 * h)
     assert _rel(got[interior(1)], fd[interior(1)]) < 1e-6
+
+
+def test_adjoint_rational_transpose_identity():
+    """<R(P), e> = <P, R^T e> for the NURBS map (linear in P, weights fixed)."""
+    rng = np.random.default_rng(33)
+    dims = [O.Dim.default(7, 3, 19), O.Dim.default(6, 2, 15), O.Dim.default(5, 2, 11)]
+    P = rng.standard_normal((7, 6, 5, 2))
+    W = rng.uniform(0.5, 2.0, (7, 6, 5))
+    E = rng.standard_normal((19, 15, 11, 2))
+    lhs = np.sum(O.eval_direct(dims, P, W=W) * E)
+    rhs = np.sum(P * O.adjoint_rational(dims, W, E))
+    assert abs(lhs - rhs) < 1e-12 * max(abs(lhs), 1.0)
+    # unit weights: the B-spline adjoint
+    np.testing.assert_allclose(O.adjoint_rational(dims, np.ones_like(W), E),
+                               O.adjoint_direct(dims, E), rtol=1e-13, atol=1e-13)
