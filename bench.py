@@ -7,7 +7,7 @@ knots, control points N(0, 1) in FP32, derivative fields (0,0), (1,0),
 (0,1), (1,1) evaluated in one fused launch.  value = sample-points/s =
 fields * samples / time.  Inputs are resident in HBM before timing starts.
 
-Prints one JSON line.  Other workloads (--workload c1|c3|c3n|c3d|c4|c4a|c5)
+Prints one JSON line.  Other workloads (--workload c1|c3|c3n|c3d|c3a|c4|c4a|c5)
 are for profiling and parity runs.
 """
 from __future__ import annotations
@@ -73,6 +73,14 @@ def workload(name: str):
                     n=(64, 64, 32), p=(3, 3, 2), m=(512, 512, 256), D=(0, 0, 0), nout=3,
                     dtype=np.float32, knots=None, fields=[(0, 0, 0)], P=P,
                     W=W if name == "c3" else None)
+    if name == "c3a":  # adjoint of the NURBS map in the control points (an extension)
+        rng = np.random.default_rng(SEED0 + 3)
+        P = rng.uniform(-1, 1, (64, 64, 32, 3)).astype(np.float32)
+        W = rng.uniform(0.5, 2.0, (64, 64, 32)).astype(np.float32)
+        return dict(name="c3a: adjoint of the 3-D->3 NURBS map (weights fixed), n=(64,64,32), "
+                         "p=(3,3,2), m=(512,512,256), FP32",
+                    n=(64, 64, 32), p=(3, 3, 2), m=(512, 512, 256), D=(0, 0, 0), nout=3,
+                    dtype=np.float32, knots=None, fields=[(0, 0, 0)], P=P, W=W, adjoint=True)
     if name == "c3d":  # gradient of the NURBS map (quotient rule, an extension)
         rng = np.random.default_rng(SEED0 + 3)
         P = rng.uniform(-1, 1, (64, 64, 32, 3)).astype(np.float32)
@@ -164,8 +172,8 @@ def algorithmic_bytes(wl, adjoint=False) -> int:
     F, nout = len(wl["fields"]), wl["nout"]
     tables = sum(m * (p + 1) * (D + 1) * s + 4 * m
                  for m, p, D in zip(wl["m"], wl["p"], wl["D"]))
-    if adjoint:
-        return nsamp * nout * s + ncp * nout * s + tables
+    if adjoint:  # (NURBS: the weights are read too)
+        return nsamp * nout * s + ncp * nout * s + tables + (ncp * s if wl["W"] is not None else 0)
     b = F * nsamp * nout * s + ncp * nout * s + tables
     if wl["W"] is not None:
         b += ncp * s
@@ -271,7 +279,10 @@ def fp64_adjoint_errors(sg, grid, ebar, pbar):
 
     g64 = fp64_grid(sg, grid)
     ref = sg.colmajor_zeros(tuple(pbar.shape), torch.float64, "cuda")
-    sg.evaluate_adjoint_(g64, eval=sg.colmajor(ebar.double()), control_points=ref)
+    if g64.is_nurbs:
+        sg.evaluate_rational_adjoint_(g64, eval=sg.colmajor(ebar.double()), control_points=ref)
+    else:
+        sg.evaluate_adjoint_(g64, eval=sg.colmajor(ebar.double()), control_points=ref)
     d = pbar.double() - ref
     scale = float(ref.abs().max()) or 1.0
     return [float(d.abs().max()) / scale], [float(d.pow(2).mean().sqrt()) / scale]
@@ -481,6 +492,8 @@ def main(argv=None):
         def step():
             if wl.get("rational"):
                 sg.evaluate_rational_fields_(grid, fields, outs)
+            elif adjoint and grid.is_nurbs:
+                sg.evaluate_rational_adjoint_(grid, eval=ebar, control_points=pbar)
             elif adjoint:
                 sg.evaluate_adjoint_(grid, eval=ebar, control_points=pbar, tier=tier)
             elif len(fields) == 1:
@@ -529,7 +542,7 @@ def main(argv=None):
             sg, grid, ebar, pbar)
     # the other arithmetic tiers on the same inputs, timed the same way
     if (wl["dtype"] == np.float32 and args.tier == "native" and not args.no_tiers
-            and not wl.get("rational")):
+            and not wl.get("rational") and not (adjoint and wl["W"] is not None)):
         tiers = {"native": {"ms_per_step": ms, "value": value,
                             "max_rel_err_vs_fp64": out["max_rel_err_vs_fp64"],
                             "rms_rel_err_vs_fp64": out["rms_rel_err_vs_fp64"]}}
@@ -548,10 +561,13 @@ def main(argv=None):
         # forward (evaluate!) plus backward (evaluate_adjoint!) per step
         def fwd_bwd():
             sg.evaluate_(grid)
-            sg.evaluate_adjoint_(grid, eval=ebar, control_points=pbar)
+            if grid.is_nurbs:
+                sg.evaluate_rational_adjoint_(grid, eval=ebar, control_points=pbar)
+            else:
+                sg.evaluate_adjoint_(grid, eval=ebar, control_points=pbar)
         out["forward_plus_adjoint_ms"], _ = timed_steps(fwd_bwd, max(20, args.steps // 2),
                                                         args.warmup)
-    if not args.no_cpu_baseline and not wl.get("rational"):
+    if not args.no_cpu_baseline and not wl.get("rational") and not (adjoint and wl["W"] is not None):
         # (the reference has no NURBS derivatives to time)
         out["cpu_baseline"] = cpu_baseline(wl, args.cpu_budget)
         out["cpu_baseline"]["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
