@@ -469,6 +469,7 @@ struct sg_batch {
   int* counters_m = nullptr;
   int n_tiles_m = 0;
   size_t lds_m = 0;
+  std::vector<int> separate_m;  // grids whose windows exceed the tier batch's LDS cap
   int n_tiles = 0;
   size_t lds = 0;
   int rows_per_wave = 16;
@@ -561,17 +562,33 @@ void pack_queues(const std::vector<std::vector<BTile>>& grid_tiles,
 
 // MFMA tiers: argument records and tiles of every grid (the grids' arrays
 // are filled into the records by sg_batch_set_arrays).  Leaves mma_ok false
-// when a grid cannot take the tiers.
+// when a grid cannot take the tiers.  A workgroup reserves at most the LDS
+// that lets kMmaBatchPerCu of them share a CU; grids whose windows need more
+// are evaluated with their own tier launches after the persistent one.
+constexpr int kMmaBatchPerCu = 4;
+
 int build_mma_tiles(sg_batch* b) {
   b->mma_ok = false;
   for (Plan* pl : b->plans)
     if (!mma_supported(*pl)) return SG_OK;
   const size_t rec = mma_args_size();
+  const size_t cap = lds_cap(kMmaBatchPerCu);
   std::vector<std::vector<BTile>> grid_tiles;
   std::vector<long long> grid_cost;
   size_t lds = 16;
+  b->separate_m.clear();
   for (size_t gi = 0; gi < b->plans.size(); ++gi) {
     const Plan& pl = *b->plans[gi];
+    size_t need = 0;
+    for (int c0 = 0; c0 < pl.nout;) {
+      const int nc = std::min(pl.nout - c0, pl.nurbs ? 3 : 4);
+      need = std::max(need, mma_lds(pl, nc + (pl.nurbs ? 1 : 0), 1));
+      c0 += nc;
+    }
+    if (need > cap) {
+      b->separate_m.push_back(static_cast<int>(gi));
+      continue;
+    }
     const Tiling& t = pl.mma;
     const int m1 = pl.dims[0].m, m2 = pl.dims[1].m;
     const int m3 = pl.nin == 3 ? pl.dims[2].m : 1;
@@ -864,6 +881,13 @@ int sg_batch_evaluate(sg_batch* b, int32_t tier, void* stream) {
       const int blocks = std::max(1, std::min(b->n_tiles_m, per_cu * b->n_cu));
       e = launch_batch_mma(tier, b->dmargs, b->dtiles_m, b->counters_m, blocks, b->lds_m, s);
     }
+    const int32_t zero[SG_MAX_NIN] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (size_t k = 0; e == hipSuccess && k < b->separate_m.size(); ++k) {
+      const int i = b->separate_m[k];
+      float* outs[1] = {static_cast<float*>(b->out[i])};
+      e = launch_forward_mma(*b->plans[i], tier, 1, zero, static_cast<const float*>(b->P[i]),
+                             static_cast<const float*>(b->W[i]), outs, s);
+    }
     if (prev != b->device) (void)hipSetDevice(prev);
     return e == hipSuccess ? SG_OK : hip_fail(e, "batch MFMA launch");
   }
@@ -968,7 +992,8 @@ int sg_batch_describe(const sg_batch* b, char* text, int32_t capacity) {
      << " samples=" << b->n_samples << " rows_per_wave=" << b->rows_per_wave
      << " blocks_per_cu=" << b->blocks_per_cu << " resident_per_cu=" << resident
      << " lds_bytes=" << b->lds << " cus=" << b->n_cu << " mfma_tiers=" << (b->mma_ok ? 1 : 0)
-     << " mfma_tiles=" << b->n_tiles_m << " mfma_lds_bytes=" << b->lds_m;
+     << " mfma_tiles=" << b->n_tiles_m << " mfma_lds_bytes=" << b->lds_m
+     << " mfma_separate=" << b->separate_m.size();
   const std::string s = os.str();
   const size_t n = std::min(s.size(), static_cast<size_t>(capacity - 1));
   std::memcpy(text, s.data(), n);
