@@ -27,10 +27,10 @@
 //
 // One wave is one task (64 samples x a chunk of up to 64 rows x a range of
 // samples j3, every channel in turn); waves share nothing, so there is no
-// LDS and no barrier.  Holding one channel's T2 and ring at a time keeps
-// registers for three waves per SIMD; a wave keeps kBufs - 1 row blocks of
-// ebar in flight ahead of the one it multiplies, across row blocks,
-// samples j3 and channels.
+// LDS and no barrier.  Only one channel's T2 and ring are live at a time;
+// the ebar buffers take most registers (two waves per SIMD): a wave keeps
+// kBufs - 1 row blocks of ebar in flight ahead of the one it multiplies,
+// across row blocks, samples j3 and channels.
 #include "sg_mma_ops.hpp"
 
 namespace sg {
