@@ -799,6 +799,7 @@ int sg_batch_create(const sg_grid_desc* grids, int32_t n_grids, void* stream, sg
   int rc = build_tiles(b, 16, 3);
   if (rc != SG_OK) return bail(rc);
   if (n_grids > 0 && (rc = build_mma_tiles(b)) != SG_OK) return bail(rc);
+  if (n_grids == 0) b->mma_ok = true;  // nothing to evaluate in any tier
   b->P.assign(n_grids, nullptr);
   b->W.assign(n_grids, nullptr);
   b->out.assign(n_grids, nullptr);

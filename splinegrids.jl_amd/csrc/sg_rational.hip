@@ -54,22 +54,28 @@ __global__ void homogenize(const T* P, const T* W, T* H, long long ncp, int nout
 // doubles (w, then R), indexed at run time without scratch memory.
 constexpr int kCombineThreads = 256;
 
-template <typename T>
+template <typename T, int K>
 __global__ __launch_bounds__(kCombineThreads) void rational_combine(RatArgs<T> a) {
   extern __shared__ double sh[];
   constexpr int S = kCombineThreads;  // stride between a thread's entries
-  const int K = a.nclosure;
   double* w = sh + threadIdx.x;
   double* R = sh + static_cast<size_t>(K) * S + threadIdx.x;
   const long long fstride = a.nsamp * (a.nout + 1);
   for (long long j = static_cast<long long>(blockIdx.x) * S + threadIdx.x; j < a.nsamp;
        j += static_cast<long long>(gridDim.x) * S) {
-    for (int g = 0; g < K; ++g)
-      w[g * S] = static_cast<double>(a.F[g * fstride + a.nout * a.nsamp + j]);
+    // the closure size is compile-time: each group of K loads is issued at once
+    T v[K];
+#pragma unroll
+    for (int g = 0; g < K; ++g) v[g] = a.F[g * fstride + a.nout * a.nsamp + j];
+#pragma unroll
+    for (int g = 0; g < K; ++g) w[g * S] = static_cast<double>(v[g]);
     for (int c = 0; c < a.nout; ++c) {
-      for (int g = 0; g < K; ++g)
-        R[g * S] = static_cast<double>(a.F[g * fstride + c * a.nsamp + j]);
+#pragma unroll
+      for (int g = 0; g < K; ++g) v[g] = a.F[g * fstride + c * a.nsamp + j];
+#pragma unroll
+      for (int g = 0; g < K; ++g) R[g * S] = static_cast<double>(v[g]);
       int t = 0;
+#pragma unroll
       for (int g = 0; g < K; ++g) {
         double acc = R[g * S];
         for (; t < a.nterms && a.tg[t] == g; ++t)
@@ -79,6 +85,15 @@ __global__ __launch_bounds__(kCombineThreads) void rational_combine(RatArgs<T> a
       for (int f = 0; f < a.nfields; ++f)
         a.out[f][c * a.nsamp + j] = static_cast<T>(R[a.out_idx[f] * S]);
     }
+  }
+}
+
+template <typename T, int K>
+void* combine_kernel_k(int k) {
+  if constexpr (K > kMaxClosure) {
+    return nullptr;
+  } else {
+    return k == K ? reinterpret_cast<void*>(rational_combine<T, K>) : combine_kernel_k<T, K + 1>(k);
   }
 }
 
@@ -239,9 +254,11 @@ hipError_t rational(const Plan& pl, int nf, const int32_t* orders, const T* P, c
   const long long blocks = std::min<long long>((nsamp + kCombineThreads - 1) / kCombineThreads,
                                                static_cast<long long>(cus) * 8);
   const size_t lds = 2 * static_cast<size_t>(K) * kCombineThreads * sizeof(double);
-  hipLaunchKernelGGL(rational_combine<T>, dim3(static_cast<unsigned>(std::max(1LL, blocks))),
-                     dim3(kCombineThreads), lds, s, a);
-  return hipGetLastError();
+  void* k = combine_kernel_k<T, 1>(K);
+  if (!k) return hipErrorInvalidValue;
+  void* args[] = {&a};
+  return hipLaunchKernel(k, dim3(static_cast<unsigned>(std::max(1LL, blocks))),
+                         dim3(kCombineThreads), args, lds, s);
 }
 
 }  // namespace
