@@ -411,7 +411,9 @@ __device__ __attribute__((always_inline)) void batch_tile_p2(const BGrid& g, con
   }
 }
 
-__global__ __launch_bounds__(kThreads) void batch_persistent(const BGrid* __restrict__ grids,
+// At most 80 VGPRs (six waves per SIMD, without spills): with the LDS share
+// of six workgroups per CU, all six stay resident.
+__global__ __launch_bounds__(kThreads, 6) void batch_persistent(const BGrid* __restrict__ grids,
                                                              const BTile* __restrict__ tiles,
                                                              int* counters) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
