@@ -254,8 +254,10 @@ __device__ __attribute__((always_inline)) void batch_mma_dispatch(const MmaArgs&
   }
 }
 
+// At most 128 VGPRs: four workgroups per CU, as many as the batch's LDS cap
+// allows (the BF16 / FP16 register demand otherwise holds it to three).
 template <int TIER>
-__global__ __launch_bounds__(kThreads) void batch_mma(const MmaArgs* __restrict__ args,
+__global__ __launch_bounds__(kThreads, 4) void batch_mma(const MmaArgs* __restrict__ args,
                                                       const BTile* __restrict__ tiles,
                                                       int* counters) {
   extern __shared__ __align__(16) unsigned char smem[];
