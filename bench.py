@@ -1,13 +1,15 @@
 """Benchmark of the grid-evaluation hot path on one MI355X.
 
 Default workload (one "step" = one pass of the hot path over one batch):
-the 2-D -> 1 freeform surface, 128 x 128 control points, degree (3, 3),
-4096 x 4096 samples, non-uniform clamped knots with 10 % doubled interior
-knots, control points N(0, 1) in FP32, derivative fields (0,0), (1,0),
-(0,1), (1,1) evaluated in one fused launch.  value = sample-points/s =
-fields * samples / time.  Inputs are resident in HBM before timing starts.
+the 3-D -> 3 NURBS grid, 64 x 64 x 32 control points, degree (3, 3, 2),
+512 x 512 x 256 samples, clamped uniform knots, weights U[0.5, 2), control
+points U[-1, 1) in FP32, one evaluate! (derivative order 0) with the
+contraction in plain FP32 (the headline value), then the split-FP16 (hi+lo)
+and BF16 MFMA tiers timed the same way on the same inputs, each with its
+max and RMS error against this library's FP64 path.  value = sample-points/s
+= samples / time.  Inputs are resident in HBM before timing starts.
 
-Prints one JSON line.  Other workloads (--workload c1|c3|c3n|c3d|c3a|c4|c4a|c5)
+Prints one JSON line.  Other workloads (--workload c1|c2|c3n|c3d|c3a|c4|c4a|c5)
 are for profiling and parity runs.
 """
 from __future__ import annotations
@@ -184,11 +186,46 @@ def algorithmic_bytes(wl, adjoint=False) -> int:
 # CPU baseline: the oracle's restatement of the reference algorithm
 # --------------------------------------------------------------------------
 
-def cpu_baseline(wl, budget_s: float = 10.0):
+def host_cpus():
+    """(threads to use, description): every CPU this process may run on, bounded
+    by the cgroup's CPU quota when one is set (a container's share of a larger
+    host), plus nproc, os.cpu_count() and the CPU model for the record."""
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = os.cpu_count() or 1
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, period = f.read().split()[:2]
+        if q != "max":
+            quota = max(1, int(int(q) // int(period)))
+    except (OSError, ValueError):
+        pass
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    threads = max(1, min(affinity, quota) if quota else affinity)
+    return threads, {"nproc": affinity, "os_cpu_count": os.cpu_count(),
+                     "cgroup_cpu_quota": quota, "cpu_model": model}
+
+
+def cpu_baseline(wl, budget_s: float = 16.0):
+    """The reference algorithm (the oracle's restatement of the direct
+    (p+1)^Nin sum per sample, src/spline_grid.jl:129-183) on the host: all
+    usable cores in the workload's value type (the headline leg), one core,
+    and all cores in FP64, each on slabs of the slowest sample axis for a
+    share of ``budget_s``."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
-    nthreads = max(1, min(16, os.cpu_count() or 1))
+    nthreads, host = host_cpus()
     dims = []
     for i in range(len(wl["n"])):
         if wl["knots"] is not None:
@@ -199,23 +236,38 @@ def cpu_baseline(wl, budget_s: float = 10.0):
             knots = O.expand_knots(v.astype(np.float64), mu)
         t = O.sample_points((knots[0], knots[-1]), wl["m"][i], wl["dtype"]).astype(np.float64)
         dims.append(O.Dim(knots, t, wl["p"][i], wl["D"][i]))
-    P = wl["P"].astype(wl["dtype"])
-    W = None if wl["W"] is None else wl["W"].astype(wl["dtype"])
-    slab_rows = max(1, min(wl["m"][-1], 512 * 4096 // max(1, math.prod(wl["m"][:-1]))))
-    done, t0 = 0, time.perf_counter()
-    start = 0
-    while time.perf_counter() - t0 < budget_s:
-        stop = min(start + slab_rows, wl["m"][-1])
-        for f in wl["fields"]:
-            O.eval_direct(dims, P, deriv=f, W=W, dtype=wl["dtype"],
-                          last_range=(start, stop), nthreads=nthreads)
-            done += math.prod(wl["m"][:-1]) * (stop - start)
-        start = 0 if stop >= wl["m"][-1] else stop
-    dt = time.perf_counter() - t0
-    return {"value": done / dt, "unit": "sample-pts/s", "cores": nthreads, "kind": "port",
-            "sample": f"reference algorithm (direct (p+1)^Nin sum per sample, "
-                      f"{np.dtype(wl['dtype']).name}) over slabs of {slab_rows} rows of the "
-                      f"slowest sample axis, all fields, {done} sample-points in {dt:.1f} s"}
+    row = math.prod(wl["m"][:-1])
+
+    def leg(dtype, threads, seconds):
+        P = wl["P"].astype(dtype)
+        W = None if wl["W"] is None else wl["W"].astype(dtype)
+        slab_rows = max(1, min(wl["m"][-1], 256 * threads * 4096 // max(1, row)))
+        done, start, t0 = 0, 0, time.perf_counter()
+        while time.perf_counter() - t0 < seconds:
+            stop = min(start + slab_rows, wl["m"][-1])
+            for f in wl["fields"]:
+                O.eval_direct(dims, P, deriv=f, W=W, dtype=dtype,
+                              last_range=(start, stop), nthreads=threads)
+                done += row * (stop - start)
+            start = 0 if stop >= wl["m"][-1] else stop
+        dt = time.perf_counter() - t0
+        return done / dt, done, dt, slab_rows
+
+    v, done, dt, slab = leg(wl["dtype"], nthreads, budget_s / 2)
+    v1, done1, dt1, _ = leg(wl["dtype"], 1, budget_s / 4)
+    v64, done64, dt64, _ = leg(np.float64, nthreads, budget_s / 4)
+    tname = np.dtype(wl["dtype"]).name
+    return {"value": v, "unit": "sample-pts/s", "cores": nthreads, "kind": "port",
+            "sample": f"the reference algorithm (direct (p+1)^Nin sum per sample, {tname}) "
+                      f"restated in C, OpenMP over {nthreads} threads, on slabs of {slab} rows "
+                      f"of the slowest sample axis of the same workload and inputs, all fields: "
+                      f"{done} sample-points in {dt:.1f} s",
+            "host": host,
+            "one_core": {"value": v1, "dtype": tname,
+                         "sample": f"{done1} sample-points in {dt1:.1f} s on 1 thread"},
+            "fp64_all_cores": {"value": v64, "dtype": "float64",
+                               "sample": f"{done64} sample-points in {dt64:.1f} s on "
+                                         f"{nthreads} threads"}}
 
 
 def fp64_errors(sg, grid, wl, outs):
@@ -350,13 +402,15 @@ def settle_gc() -> None:
 
 def timed_steps(step, steps: int, warmup: int):
     """Average ms per step over `steps` back-to-back steps on the current
-    stream (HIP events), after `warmup` untimed steps; also the wall time."""
+    stream (HIP events), after `warmup` untimed steps; also the wall time.
+    The collector is settled before the warm-up, so that the GPU goes from
+    the warm-up straight into the timed steps."""
     import torch
 
+    settle_gc()
     for _ in range(warmup):
         step()
     torch.cuda.synchronize()
-    settle_gc()
     stream = torch.cuda.current_stream()
     start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     t_wall = time.perf_counter()
@@ -382,10 +436,10 @@ def bench_batch(args, sg, wl):
     info = batch.autotune(reps=3)
 
     def timed(fn, steps):
+        settle_gc()
         for _ in range(args.warmup):
             fn()
         torch.cuda.synchronize()
-        settle_gc()
         a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         a.record()
         for _ in range(steps):
@@ -455,9 +509,10 @@ def main(argv=None):
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="c2")
+    ap.add_argument("--workload", default="c3")
     ap.add_argument("--tier", default="native")
-    ap.add_argument("--cpu-budget", type=float, default=10.0)
+    ap.add_argument("--cpu-budget", type=float, default=16.0,
+                    help="seconds of CPU-baseline work over its three legs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-tiers", action="store_true",
                     help="skip timing the split-FP16 and BF16 tiers beside FP32")

@@ -210,7 +210,9 @@ int sg_batch_evaluate_each(sg_batch* batch, void* stream);
 /* Time candidate tilings (workgroups per CU {2,3,4,6} x rows per wave
  * {8,16,32}; each grid's rows are halved until its tiles fit the LDS share of
  * one workgroup) on the arrays set last and keep the fastest; `reps` timed
- * launches each. */
+ * launches each.  A graph captured from sg_batch_evaluate keeps the tiling of
+ * its capture, so once a capture has been recorded the batch refuses to
+ * retune (SG_ERR_INVALID): autotune first, then capture. */
 int sg_batch_autotune(sg_batch* batch, int32_t reps, void* stream);
 
 /* Totals of the batch: tiles, sample points, grids evaluated by the

@@ -40,10 +40,10 @@ void sgo_sample_indices(const double* t, int32_t m, const double* knots_all,
     const double s = t[l];
     int32_t count = 0;
     for (int32_t k = 0; k < n_knots; ++k) {
-      if (s < knots_all[k]) break; /* :306-311, ties go right */
+      if (s < knots_all[k]) break; /* :35-41, ties go right */
       ++count;
     }
-    const int32_t lo = p + 1, hi = n_knots - p - 1; /* :314-318 */
+    const int32_t lo = p + 1, hi = n_knots - p - 1; /* :43-47 */
     if (count < lo) count = lo;
     if (count > hi) count = hi;
     idx1[l] = count;
@@ -243,19 +243,19 @@ void sgo_eval_points_f64(int32_t nin, int32_t nout, const int32_t* n,
       for (int32_t d = 0; d < nin; ++d) {                                     \
         J[d] = (int32_t)(r % m[d]);                                           \
         r /= m[d];                                                            \
-        base[d] = (int64_t)idx1[d][J[d]] - p[d] - 1; /* :248-249 */           \
+        base[d] = (int64_t)idx1[d][J[d]] - p[d] - 1; /* :18-19 */             \
         tab[d] = tables[d] + J[d] +                                           \
                  (int64_t)m[d] * ((int64_t)(p[d] + 1) * deriv[d]);            \
         I[d] = 0;                                                             \
       }                                                                       \
-      for (int64_t term = 0; term < g.n_terms; ++term) { /* :251 */           \
+      for (int64_t term = 0; term < g.n_terms; ++term) { /* :21 */            \
         int64_t cp = 0;                                                       \
         T prod = (T)1;                                                        \
         for (int32_t d = 0; d < nin; ++d) {                                   \
           cp += (base[d] + I[d]) * g.cp_stride[d];                            \
-          prod *= tab[d][(int64_t)m[d] * I[d]]; /* :258-261 */                \
+          prod *= tab[d][(int64_t)m[d] * I[d]]; /* :28-31 */                  \
         }                                                                     \
-        for (int32_t c = 0; c < nout; ++c) { /* :264-268 */                   \
+        for (int32_t c = 0; c < nout; ++c) { /* :34-38 */                     \
           const T v = prod * ebar[q + g.n_samples * c];                       \
           ATOMIC                                                              \
           Pbar[cp + g.n_cp * c] += v;                                         \

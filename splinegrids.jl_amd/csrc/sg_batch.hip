@@ -481,6 +481,10 @@ struct sg_batch {
   long long n_samples = 0;
   std::vector<const void*> P, W;
   std::vector<void*> out;
+  // set once sg_batch_evaluate has been recorded into a stream capture: a
+  // captured launch keeps the tile buffer, grid size and LDS of the tiling
+  // at capture time, so the tiling may no longer change (autotune refuses)
+  bool captured = false;
 };
 
 namespace {
@@ -877,6 +881,9 @@ int sg_batch_evaluate(sg_batch* b, int32_t tier, void* stream) {
   (void)hipGetDevice(&prev);
   if (prev != b->device) (void)hipSetDevice(b->device);
   hipStream_t s = static_cast<hipStream_t>(stream);
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  if (s && hipStreamIsCapturing(s, &cap) == hipSuccess && cap != hipStreamCaptureStatusNone)
+    b->captured = true;
   if (tier != SG_TIER_NATIVE) {
     hipError_t e = hipSuccess;
     if (b->n_tiles_m > 0) {
@@ -927,6 +934,11 @@ int sg_batch_evaluate_each(sg_batch* b, void* stream) {
 
 int sg_batch_autotune(sg_batch* b, int32_t reps, void* stream) {
   if (!b) return fail(SG_ERR_INVALID, "batch is NULL");
+  if (b->captured)
+    return fail(SG_ERR_INVALID,
+                "sg_batch_autotune after sg_batch_evaluate was captured into a graph: the "
+                "captured launch keeps the tiling it was recorded with; autotune before "
+                "capturing");
   for (size_t i = 0; i < b->out.size(); ++i)
     if (!b->out[i] && b->hgrids[i].nsamp > 0)
       return fail(SG_ERR_INVALID, "sg_batch_set_arrays was not called");

@@ -13,6 +13,7 @@
 // lane gets the product whatever the hardware's K order.
 #pragma once
 
+#include "sg_bf16.hpp"
 #include "sg_internal.hpp"
 
 namespace sg {
@@ -23,10 +24,8 @@ typedef float float4_t __attribute__((ext_vector_type(4)));
 typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 
-__device__ inline short bf16_bits(float x) {  // round to nearest even
-  unsigned u = __float_as_uint(x);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return static_cast<short>(u >> 16);
+__device__ inline short bf16_bits(float x) {  // round to nearest even, NaN stays NaN
+  return static_cast<short>(bf16_round_bits(__float_as_uint(x)));
 }
 
 // The largest v over the wave (v >= 0), as a wave-uniform value.  DPP row

@@ -59,13 +59,17 @@ class SplineDimension:
             host_t = np.asarray(sample_points, dtype=ft)
             assert len(host_t) == n_sample_points
         self._host_t = host_t
+        self._host_knots = np.asarray(knot_vector.knots_all, dtype=np.float64).copy()
+        # bumped by every evaluate_dimension_: grids rebuild their plans when
+        # a dimension's sample points or knots have been re-evaluated
+        self.version = 0
         self.sample_points = torch.from_numpy(host_t.copy()).to(self.device)
         self._knots_dev = torch.from_numpy(
             np.asarray(knot_vector.knots_all, dtype=ft).copy()).to(self.device)
         m, w, nd = int(n_sample_points), self.degree + 1, self.max_derivative_order + 1
         self.sample_indices = torch.zeros(m, dtype=torch.int32, device=self.device)
         self.eval = colmajor_zeros((m, w, nd), self.float_type, self.device)
-        evaluate_dimension_(self)
+        evaluate_dimension_(self, _refresh=False)
 
     # ---- size helpers, src/utils.jl:31-56 ----
     @property
@@ -77,7 +81,8 @@ class SplineDimension:
         return len(self._host_t)
 
     def host_knots(self) -> np.ndarray:
-        return np.asarray(self.knot_vector.knots_all, dtype=np.float64)
+        """knots_all as of the last evaluation of this dimension (FP64)."""
+        return self._host_knots.copy()
 
     def host_sample_points(self) -> np.ndarray:
         return self._host_t.astype(np.float64)
@@ -89,10 +94,24 @@ class SplineDimension:
                 f"float_type={self.float_type})")
 
 
-def evaluate_dimension_(s: SplineDimension) -> None:
+def evaluate_dimension_(s: SplineDimension, _refresh: bool = True) -> None:
     """set_sample_indices! + evaluate!(::SplineDimension)
-    (src/utils.jl:19-29, src/spline_dimension.jl:231-242) on the GPU."""
+    (src/utils.jl:19-29, src/spline_dimension.jl:231-242) on the GPU.
+
+    As in the reference, a caller may edit ``sample_points`` (the device
+    vector) or the knot vector's ``knots_all`` and re-evaluate: the call
+    takes the current values of both, and grids built on this dimension
+    rebuild their plans (tables, tiling) on their next evaluation."""
     L = _lib.lib()
+    if _refresh:
+        knots = np.asarray(s.knot_vector.knots_all)
+        if len(knots) != len(s._knots_dev):
+            raise ValueError("the number of knots of a dimension cannot change; "
+                             "build a new SplineDimension")
+        s._knots_dev.copy_(torch.from_numpy(np.asarray(knots, dtype=s._host_t.dtype).copy()))
+        s._host_t = s.sample_points.detach().cpu().numpy().astype(s._host_t.dtype)
+        s._host_knots = np.asarray(knots, dtype=np.float64).copy()
+        s.version += 1
     with torch.cuda.device(s.device):
         _lib.check(L.sg_dimension_evaluate(
             s._knots_dev.data_ptr(), len(s._knots_dev), s.sample_points.data_ptr(),

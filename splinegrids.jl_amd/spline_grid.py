@@ -112,6 +112,7 @@ class SplineGrid:
         self.weights = weights
         self.eval = eval
         self._plan: Optional[Plan] = None
+        self._plan_key = None
 
     @property
     def Nin(self) -> int:
@@ -133,9 +134,17 @@ class SplineGrid:
     def dtype(self):
         return obtain(self.control_points).dtype
 
+    def _dims_key(self):
+        return tuple((id(s), s.version) for s in self.spline_dimensions)
+
     def plan(self) -> Plan:
-        if self._plan is None:
+        """The C-ABI plan of this grid, rebuilt when a dimension has been
+        re-evaluated (new sample points or knots) since it was built."""
+        key = self._dims_key()
+        if self._plan is None or self._plan_key != key:
+            self._plan = None
             self._plan = Plan(self.spline_dimensions, self.Nout, self.is_nurbs, self.device)
+            self._plan_key = key
         return self._plan
 
     def __repr__(self):

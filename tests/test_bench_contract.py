@@ -23,6 +23,18 @@ def test_bench_refuses_several_gpus(capsys):
     assert "one GPU only" in capsys.readouterr().err
 
 
+def test_cpu_baseline_legs():
+    """The CPU baseline reports the threads it used, the host (nproc, CPU
+    model, cgroup quota), a one-core leg and an FP64 leg."""
+    wl = bench.workload("c1")
+    cb = bench.cpu_baseline(wl, budget_s=0.4)
+    assert cb["kind"] == "port" and cb["unit"] == "sample-pts/s" and cb["value"] > 0
+    threads, host = bench.host_cpus()
+    assert cb["cores"] == threads >= 1
+    assert {"nproc", "os_cpu_count", "cgroup_cpu_quota", "cpu_model"} <= set(cb["host"])
+    assert cb["one_core"]["value"] > 0 and cb["fp64_all_cores"]["dtype"] == "float64"
+
+
 @pytest.mark.gpu
 def test_bench_default_line(gpu):
     buf = io.StringIO()
@@ -36,8 +48,8 @@ def test_bench_default_line(gpu):
     assert ROOFLINE <= set(d["roofline"])
     assert d["n_gpus"] == 1 and d["steps"] == 5 and d["warmup"] == 1
     assert d["unit"] == "sample-pts/s" and d["higher_is_better"] is True
-    assert d["dtype"] == "f32" and d["config"]["workload"].startswith("c2")
+    assert d["dtype"] == "f32" and d["config"]["workload"].startswith("c3:")
     assert 0 < d["roofline"]["frac"] < 1
-    assert d["value"] == pytest.approx(4 * 4096 * 4096 / (d["ms_per_step"] * 1e-3), rel=1e-9)
+    assert d["value"] == pytest.approx(512 * 512 * 256 / (d["ms_per_step"] * 1e-3), rel=1e-9)
     assert max(d["max_rel_err_vs_fp64"]) < 1e-6
     assert {"native", "split_f16", "bf16"} <= set(d["tiers"])

@@ -9,7 +9,8 @@ last tiles and chunks, 1..6 channels (channel blocks of 4), p3 = 0..5, 2-D
 grids, derivative orders, accumulation, and a grid too sparse along
 dimension 1 for the matrix-core tiles.  The split-FP16 and BF16 tiers run
 the first product of the matrix-core adjoint (ebar times B1) on FP16 / BF16
-MFMA and are held to the FP32 bar (split FP16) or 1e-2 (BF16, stated).
+MFMA and are held to the FP32 bar of 1e-6 relative to max|oracle| (split
+FP16) or 1e-2 (BF16, stated).
 """
 from __future__ import annotations
 
@@ -62,13 +63,13 @@ def test_adjoint_paths_vs_oracle(gpu, case, order, monkeypatch):
     deriv = (lambda D: tuple(0 for _ in D)) if order == "zero" else (lambda D: tuple(D))
     got_m, want, desc_m = _run(sg, case, deriv, True, monkeypatch)
     assert ("adjoint=mma_f32" in desc_m) == case[4], desc_m
-    assert rel_err(got_m, want) < 1e-5, desc_m
+    assert rel_err(got_m, want) < 1e-6, desc_m
     got_s, want_s, desc_s = _run(sg, case, deriv, False, monkeypatch)
     assert "adjm=none" in desc_s and "adjoint=stream " in desc_s, desc_s
-    assert rel_err(got_s, want_s) < 1e-5, desc_s
+    assert rel_err(got_s, want_s) < 1e-6, desc_s
 
 
-TIER_BAR = {"split_f16": 1e-5, "bf16": 1e-2}
+TIER_BAR = {"split_f16": 1e-6, "bf16": 1e-2}
 
 
 @pytest.mark.parametrize("case", [c for c in CASES if c[4]], ids=lambda c: f"n{c[0]}m{c[2]}c{c[3]}")
@@ -94,14 +95,14 @@ def test_adjoint_split_f16_tiny_and_huge_ebar(gpu, monkeypatch):
         Pbar = sg.colmajor_zeros(tuple(n) + (nout,), grid.dtype, "cuda")
         sg.evaluate_adjoint_(grid, eval=ebar, control_points=Pbar, tier="split_f16")
         want = O.adjoint_direct(oracle_dims(dims), E.astype(np.float64), nthreads=8)
-        assert rel_err(eval_host(sg, Pbar), want) < 1e-5, scale
+        assert rel_err(eval_host(sg, Pbar), want) < 1e-6, scale
 
 
 def test_adjoint_mma_accumulates(gpu, monkeypatch):
     got, want, desc = _run(gpu, CASES[0], lambda D: (0, 0, 0), True, monkeypatch,
                            accumulate=True)
     assert "adjoint=mma_f32" in desc
-    assert rel_err(got, want) < 1e-5
+    assert rel_err(got, want) < 1e-6
 
 
 def test_adjoint_mma_is_deterministic(gpu):

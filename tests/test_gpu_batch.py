@@ -105,6 +105,9 @@ def test_batch_graph_replays(gpu):
         torch.cuda.synchronize()
         for i, (g, r) in enumerate(zip(grids, ref)):
             assert torch.equal(g.eval, r), (rep, i)
+    # the captured launch keeps its tiling: retuning is refused from now on
+    with pytest.raises(sg.SplineGridsError, match="autotune before capturing"):
+        batch.autotune(reps=1)
 
 
 def test_batch_rejects_bad_input(gpu):

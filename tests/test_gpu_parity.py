@@ -81,6 +81,67 @@ def test_fd_derivatives(gpu):
     torch.testing.assert_close(B2[2:], fd2, rtol=1e-2, atol=1e-2 * fd2.abs().max().item())
 
 
+@pytest.mark.parametrize("p", [0, 1, 2, 3, 5])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_basis_samples_on_knots_and_outside_extent(gpu, p, dtype):
+    """Span-search ties and the clamp (src/util_kernels.jl:22-49): samples
+    placed exactly on every knot (interior, repeated and the end knots, so
+    ties go right) and samples left and right of the knot extent (clamped
+    to the first / last non-empty span).  Indices and FP64 tables bit-exact
+    against the oracle."""
+    sg = gpu
+    rng = np.random.default_rng(100 + p)
+    inner = np.sort(rng.uniform(0.05, 0.95, 9))
+    vals = np.concatenate([[0.0], inner, [1.0]])
+    mults = np.ones(len(vals), dtype=np.int32)
+    mults[0] = mults[-1] = p + 1
+    if p > 0:
+        mults[4] = min(p, 2)  # a repeated interior knot
+        mults[7] = p          # a knot of full interior multiplicity
+    kv = sg.KnotVector(vals.astype(dtype), mults)
+    knots = np.asarray(kv.knots_all, dtype=np.float64)
+    t = np.concatenate([knots, [-0.5, -1e-3, 1.0 + 1e-3, 1.7], rng.uniform(-0.2, 1.2, 40)])
+    t = t.astype(dtype)
+    n = len(knots) - p - 1
+    D = min(p, 2)
+    s = sg.SplineDimension(n, p, len(t), max_derivative_order=D, knot_vector=kv,
+                           sample_points=t, device="cuda")
+    od = O.Dim(s.host_knots(), s.host_sample_points(), p, D)
+    want_idx = np.clip(np.searchsorted(knots, t.astype(np.float64), side="right"), p + 1,
+                       len(knots) - p - 1)
+    assert np.array_equal(od.idx1, want_idx)  # the oracle's span rule by brute force
+    assert np.array_equal(s.sample_indices.cpu().numpy(), od.idx1)
+    assert np.array_equal(eval_host(sg, s.eval), od.table.astype(dtype))
+    grid = sg.SplineGrid(s, 1)
+    table, idx = grid.plan().basis(0)  # the plan's tables take the same path
+    assert np.array_equal(idx, od.idx1)
+    assert np.array_equal(table, od.table)
+
+
+def test_edited_sample_points_rebuild_the_plan(gpu):
+    """src/utils.jl:19-29, src/spline_dimension.jl:231-242: a caller may
+    overwrite a dimension's sample points and re-evaluate it; the next grid
+    evaluation must use the new samples (indices, tables and plan)."""
+    sg = gpu
+    rng = np.random.default_rng(31)
+    dims = make_dims(sg, (11, 9), (3, 2), (60, 45), dtype=np.float64)
+    grid = sg.SplineGrid(tuple(dims), 2)
+    P = random_cp(rng, (11, 9), 2, np.float64)
+    grid.control_points.copy_(P)
+    sg.evaluate_(grid)
+    before = grid.plan()
+    new_t = np.sort(rng.uniform(0, 1, 60)) ** 2
+    dims[0].sample_points.copy_(torch.from_numpy(new_t))
+    sg.evaluate_(dims[0])
+    od = oracle_dims(dims)
+    assert np.array_equal(od[0].t, new_t)
+    assert np.array_equal(dims[0].sample_indices.cpu().numpy(), od[0].idx1)
+    sg.evaluate_(grid)
+    assert grid.plan() is not before
+    want = O.eval_direct(od, P)
+    assert rel_err(eval_host(sg, grid.eval), want) < 1e-12
+
+
 def test_plan_tables_match_dimension(gpu):
     sg = gpu
     dims = make_dims(sg, (9, 8), (3, 2), (50, 40), D=(2, 1), dtype=np.float64)
@@ -269,7 +330,7 @@ def test_adjoint_vs_oracle(gpu, case, dtype):
         sg.evaluate_adjoint_(grid, derivative_order=deriv, eval=ebar, control_points=Pbar)
         want = O.adjoint_direct(od, E.astype(np.float64), deriv=deriv)
         err = rel_err(eval_host(sg, Pbar), want)
-        assert err < TOL[dtype] * (10 if dtype == np.float32 else 1), (deriv, err)
+        assert err < TOL[dtype], (deriv, err)
 
 
 def test_adjoint_dot_product_identity(gpu):
