@@ -42,6 +42,7 @@ extern "C" {
 /* value types of the caller's arrays (Tv in the reference) */
 #define SG_F32 0
 #define SG_F64 1
+#define SG_U8 2 /* flags (0/1) of the THB bookkeeping, where a call says so */
 
 /* arithmetic tiers for the tensor-product contraction.  Basis tables are
  * always computed in FP64. */
@@ -225,6 +226,85 @@ int sg_batch_info(const sg_batch* batch, int64_t* n_tiles, int64_t* n_samples,
  * workgroups per CU (for logs and tests), truncated to capacity-1 characters
  * and NUL-terminated. */
 int sg_batch_describe(const sg_batch* batch, char* text, int32_t capacity);
+
+/* ---- refinement matrices and locally refined (THB) control points ------
+ *
+ * A refinement matrix as the reference's RefinementMatrix stores it
+ * (src/refinement_matrix.jl:1-60): m x n, consecutive non-zeros in every row
+ * and column; 1-based device arrays row_pointer[m] and column_start[m] (int32)
+ * and the non-zeros nzval[nnz] (value type given by the call's dtype); row i
+ * holds nzval[row_pointer[i] .. row_pointer[i+1]-1] (row_pointer[m+1] =
+ * nnz + 1) at columns column_start[i], column_start[i] + 1, ...  The struct
+ * itself is host memory. */
+typedef struct {
+  int32_t m, n, nnz;
+  const int32_t* row_pointer;
+  const int32_t* column_start;
+  const void* nzval;
+} sg_refmat;
+
+/* The (n_basis + 1) x n_basis refinement matrix of inserting knot_new into
+ * knots_all (device, n_basis + degree + 1 values of dtype), knot_span_index
+ * 1-based with knots_all[k] <= knot_new < knots_all[k + 1]: replaces
+ * build_refinement_matrix_kernel and RefinementMatrix(spline_dimension,
+ * knot_span_index, knot_new) (src/refinement.jl:3-84).  Outputs (device):
+ * row_pointer, column_start [n_basis + 1] and nzval [n_basis + degree + 1].
+ * alpha is computed in FP64 and rounded to dtype. */
+int sg_refinement_insert_knot(const void* knots_all, int32_t n_basis, int32_t degree,
+                              int32_t knot_span_index, double knot_new, int32_t* row_pointer,
+                              int32_t* column_start, void* nzval, int32_t dtype, void* stream);
+
+/* C = A B (Base.:*, src/refinement_matrix.jl:273-327) in two calls: the
+ * non-zeros per row of C and their first column (device [A.m] each); the
+ * caller forms row_pointer (1 + exclusive prefix sum) and allocates nzval,
+ * then the values (C->nzval is written).  A and B must be valid
+ * (sg_refinement_validate). */
+int sg_refinement_product_rows(const sg_refmat* A, const sg_refmat* B, int32_t* n_nonzero,
+                               int32_t* column_start, void* stream);
+int sg_refinement_product_values(const sg_refmat* A, const sg_refmat* B, const sg_refmat* C,
+                                 int32_t dtype, void* stream);
+
+/* validate_refinement_matrix_kernel (src/refinement_matrix.jl:134-181): one
+ * flag per row (device uint8 [m]), 1 where the row is valid. */
+int sg_refinement_validate(const sg_refmat* A, uint8_t* valid_rows, void* stream);
+
+/* mult!(Y, As, B, dims_refinement) (src/refinement_matrix.jl:421-445): Y and B
+ * are column-major device arrays of nd dimensions (the last the output
+ * channel, never refined); dims[k] (1-based) is the dimension mats[k]
+ * refines (shape_y = mats[k].m, shape_b = mats[k].n there; other dimensions
+ * equal).  Y[I] = sum_J prod_d R_d[I_d, J_d] B[J], overwritten.  FP32/FP64. */
+int sg_refinement_mult(int32_t nd, const int32_t* shape_y, const int32_t* shape_b,
+                       int32_t n_mats, const sg_refmat* mats, const int32_t* dims, const void* B,
+                       void* Y, int32_t dtype, void* stream);
+
+/* mult_adjoint!(B, As, Y, dims_refinement) (src/adjoint.jl:127-152) without
+ * atomics: B = M^T Y (overwritten, or added to with accumulate = 1), each
+ * element of B gathering the rows that hold it in a fixed order.  dtype
+ * SG_U8 is the Flag form of deactivate_overwritten_control_points!
+ * (src/control_points.jl:624-680): B[J] = 1 where some I with Y[I] != 0
+ * holds J (structure only; the nzval are not read). */
+int sg_refinement_mult_adjoint(int32_t nd, const int32_t* shape_y, const int32_t* shape_b,
+                               int32_t n_mats, const sg_refmat* mats, const int32_t* dims,
+                               const void* Y, void* B, int32_t dtype, int32_t accumulate,
+                               void* stream);
+
+/* local_refinement_kernel (src/control_points.jl:296-311): for i < n_idx,
+ * control_points[indices[i, :]..., c] = values[i, c].  control_points has
+ * shape (shape[0..nin-1]..., nout); indices [n_idx, nin] int32 1-based and
+ * values [n_idx, nout], all column-major device arrays; F32, F64 or U8.
+ * A row of indices outside the grid is skipped. */
+int sg_local_refinement_overwrite(int32_t nin, int32_t nout, const int32_t* shape, int64_t n_idx,
+                                  const int32_t* indices, const void* values,
+                                  void* control_points, int32_t dtype, void* stream);
+
+/* values[i, c] = control_points[indices[i, :]..., c]; with clear = 1 that
+ * entry is then zeroed (local_refinement_adjoint_kernel, src/adjoint.jl:
+ * 154-170), with clear = 0 it is kept (the read of refinement_values_new_
+ * kernel, src/control_points.jl:427-456).  A row of indices outside the grid
+ * reads 0. */
+int sg_local_refinement_gather(int32_t nin, int32_t nout, const int32_t* shape, int64_t n_idx,
+                               const int32_t* indices, void* values, void* control_points,
+                               int32_t clear, int32_t dtype, void* stream);
 
 #ifdef __cplusplus
 }

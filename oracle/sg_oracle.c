@@ -289,3 +289,157 @@ void sgo_adjoint_direct_f32(int32_t nin, int32_t nout, const int32_t* n,
   (void)D;
   adjoint_f32(nin, nout, n, m, p, tables, idx1, deriv, ebar, Pbar, nthreads);
 }
+
+/* ---- refinement matrices and THB control points ---- */
+
+/* 1-based column range of row i (1-based) of a refinement matrix:
+ * src/refinement_matrix.jl:103-131 (get_column_end, get_column_range). */
+static void refmat_row(const sgo_refmat* A, int32_t i, int32_t* cs, int32_t* ce) {
+  const int32_t rp_next = (i == A->m) ? A->nnz + 1 : A->row_pointer[i];
+  const int32_t rp = A->row_pointer[i - 1];
+  *cs = A->column_start[i - 1];
+  *ce = *cs + (rp_next - rp) - 1;
+}
+
+typedef struct {
+  int32_t nd;
+  int64_t stride_y[SGO_MAX_NIN + 1], stride_b[SGO_MAX_NIN + 1];
+  int64_t size_y;
+} refmat_geom;
+
+static void refmat_geom_init(refmat_geom* g, int32_t nd, const int32_t* shape_y,
+                             const int32_t* shape_b) {
+  g->nd = nd;
+  int64_t sy = 1, sb = 1;
+  for (int32_t d = 0; d < nd; ++d) {
+    g->stride_y[d] = sy;
+    g->stride_b[d] = sb;
+    sy *= shape_y[d];
+    sb *= shape_b[d];
+  }
+  g->size_y = sy;
+}
+
+/* For Y index I (0-based, from linear q): per dimension the first column
+ * (1-based), the number of columns and the first nzval position (1-based,
+ * 0 for an unrefined dimension): src/utils.jl:204-234 (get_row_extends). */
+static void refmat_extends(const refmat_geom* g, const int32_t* shape_y, const sgo_refmat* mats,
+                           const int32_t* refmat_of, int64_t q, int32_t* I, int32_t* col0,
+                           int32_t* ncol, int32_t* rp) {
+  int64_t r = q;
+  for (int32_t d = 0; d < g->nd; ++d) {
+    I[d] = (int32_t)(r % shape_y[d]);
+    r /= shape_y[d];
+    if (refmat_of[d] == 0) {
+      col0[d] = I[d] + 1;
+      ncol[d] = 1;
+      rp[d] = 0;
+    } else {
+      const sgo_refmat* A = &mats[refmat_of[d] - 1];
+      int32_t cs, ce;
+      refmat_row(A, I[d] + 1, &cs, &ce);
+      col0[d] = cs;
+      ncol[d] = ce - cs + 1;
+      rp[d] = A->row_pointer[I[d]];
+    }
+  }
+}
+
+#define REFMAT_WALK(BODY)                                                      \
+  for (int64_t q = 0; q < g.size_y; ++q) {                                     \
+    int32_t I[SGO_MAX_NIN + 1], col0[SGO_MAX_NIN + 1], ncol[SGO_MAX_NIN + 1],  \
+        rp[SGO_MAX_NIN + 1], Jb[SGO_MAX_NIN + 1];                              \
+    refmat_extends(&g, shape_y, mats, refmat_of, q, I, col0, ncol, rp);        \
+    int64_t nterms = 1;                                                        \
+    for (int32_t d = 0; d < nd; ++d) {                                         \
+      nterms *= ncol[d];                                                       \
+      Jb[d] = 0;                                                               \
+    }                                                                          \
+    for (int64_t t = 0; t < nterms; ++t) {                                     \
+      int64_t jb = 0;                                                          \
+      for (int32_t d = 0; d < nd; ++d) jb += (int64_t)(col0[d] - 1 + Jb[d]) * g.stride_b[d]; \
+      BODY                                                                     \
+      for (int32_t d = 0; d < nd; ++d) { /* column-major odometer */           \
+        if (++Jb[d] < ncol[d]) break;                                          \
+        Jb[d] = 0;                                                             \
+      }                                                                        \
+    }                                                                          \
+  }
+
+void sgo_refmat_mult(int32_t nd, const int32_t* shape_y, const int32_t* shape_b,
+                     const sgo_refmat* mats, const int32_t* refmat_of, const double* B,
+                     double* Y) {
+  refmat_geom g;
+  refmat_geom_init(&g, nd, shape_y, shape_b);
+  for (int64_t q = 0; q < g.size_y; ++q) Y[q] = 0.0; /* :386 out = zero */
+  REFMAT_WALK({
+    double contrib = B[jb]; /* :391 */
+    for (int32_t d = 0; d < nd; ++d)
+      if (refmat_of[d] != 0) /* :392-398 */
+        contrib *= mats[refmat_of[d] - 1].nzval[rp[d] + Jb[d] - 1];
+    Y[q] += contrib; /* :399-402 */
+  })
+}
+
+void sgo_refmat_mult_adjoint(int32_t nd, const int32_t* shape_y, const int32_t* shape_b,
+                             const sgo_refmat* mats, const int32_t* refmat_of,
+                             const double* Y, double* B) {
+  refmat_geom g;
+  refmat_geom_init(&g, nd, shape_y, shape_b);
+  int64_t size_b = 1;
+  for (int32_t d = 0; d < nd; ++d) size_b *= shape_b[d];
+  for (int64_t k = 0; k < size_b; ++k) B[k] = 0.0; /* src/adjoint.jl:135 */
+  REFMAT_WALK({
+    double contrib = Y[q]; /* :109 */
+    for (int32_t d = 0; d < nd; ++d)
+      if (refmat_of[d] != 0) /* :110-116 */
+        contrib *= mats[refmat_of[d] - 1].nzval[rp[d] + Jb[d] - 1];
+    B[jb] += contrib; /* :122 (atomic in the reference) */
+  })
+}
+
+void sgo_refmat_mult_adjoint_flags(int32_t nd, const int32_t* shape_y, const int32_t* shape_b,
+                                   const sgo_refmat* mats, const int32_t* refmat_of,
+                                   const uint8_t* Y, uint8_t* B) {
+  refmat_geom g;
+  refmat_geom_init(&g, nd, shape_y, shape_b);
+  int64_t size_b = 1;
+  for (int32_t d = 0; d < nd; ++d) size_b *= shape_b[d];
+  for (int64_t k = 0; k < size_b; ++k) B[k] = 0;
+  REFMAT_WALK({
+    if (Y[q]) B[jb] = 1; /* src/adjoint.jl:117-120: a true Flag is written */
+  })
+}
+
+void sgo_local_refinement_overwrite(int32_t nin, int32_t nout, const int32_t* shape,
+                                    int64_t n_idx, const int32_t* indices,
+                                    const double* values, double* cp) {
+  int64_t ncp = 1;
+  for (int32_t d = 0; d < nin; ++d) ncp *= shape[d];
+  for (int64_t i = 0; i < n_idx; ++i) {
+    int64_t lin = 0, stride = 1;
+    for (int32_t d = 0; d < nin; ++d) {
+      lin += (int64_t)(indices[i + n_idx * d] - 1) * stride; /* :305 */
+      stride *= shape[d];
+    }
+    for (int32_t c = 0; c < nout; ++c) cp[lin + ncp * c] = values[i + n_idx * c]; /* :307-310 */
+  }
+}
+
+void sgo_local_refinement_gather(int32_t nin, int32_t nout, const int32_t* shape,
+                                 int64_t n_idx, const int32_t* indices, double* values,
+                                 double* cp) {
+  int64_t ncp = 1;
+  for (int32_t d = 0; d < nin; ++d) ncp *= shape[d];
+  for (int64_t i = 0; i < n_idx; ++i) {
+    int64_t lin = 0, stride = 1;
+    for (int32_t d = 0; d < nin; ++d) {
+      lin += (int64_t)(indices[i + n_idx * d] - 1) * stride; /* :164 */
+      stride *= shape[d];
+    }
+    for (int32_t c = 0; c < nout; ++c) { /* :166-169 */
+      values[i + n_idx * c] = cp[lin + ncp * c];
+      cp[lin + ncp * c] = 0.0;
+    }
+  }
+}

@@ -92,6 +92,57 @@ void sgo_adjoint_direct_f32(int32_t nin, int32_t nout, const int32_t* n,
                             const int32_t* const* idx1, const int32_t* deriv,
                             const float* ebar, float* Pbar, int32_t nthreads);
 
+/* ---- refinement matrices and THB control points ----
+ *
+ * A refinement matrix (src/refinement_matrix.jl:1-60) as the reference
+ * stores it, 1-based: row i (1..m) holds nzval[row_pointer[i] ..
+ * row_pointer[i+1] - 1] (row_pointer[m+1] = nnz + 1) at columns
+ * column_start[i], column_start[i] + 1, ... */
+typedef struct {
+  int32_t m, n, nnz;
+  const int32_t* row_pointer;
+  const int32_t* column_start;
+  const double* nzval;
+} sgo_refmat;
+
+/* mult! (src/refinement_matrix.jl:421-445, kernel :365-403): Y and B have
+ * nd dimensions (the last one the output channel); refmat_of[d] is 0 when
+ * dimension d is not refined (then shape_y[d] == shape_b[d]), else the
+ * 1-based index into mats of its refinement matrix.  For every Y index I,
+ * Y[I] = sum over the column ranges J of prod_d R_d[I_d, J_d] * B[J], the
+ * ranges walked first dimension fastest and the product taken B[J] first,
+ * then dimension 1, 2, ... */
+void sgo_refmat_mult(int32_t nd, const int32_t* shape_y, const int32_t* shape_b,
+                     const sgo_refmat* mats, const int32_t* refmat_of, const double* B,
+                     double* Y);
+
+/* mult_adjoint! (src/adjoint.jl:127-152, kernel :85-125): B zeroed, then
+ * every Y index scatters prod_d R_d[I_d, J_d] * Y[I] into B[J]. */
+void sgo_refmat_mult_adjoint(int32_t nd, const int32_t* shape_y, const int32_t* shape_b,
+                             const sgo_refmat* mats, const int32_t* refmat_of,
+                             const double* Y, double* B);
+
+/* The Flag form of the same scatter (deactivate_overwritten_control_points!,
+ * src/control_points.jl:624-680 with Flag, src/utils.jl:236-247): B[J] is
+ * set to 1 when some I with Y[I] != 0 holds J in its column ranges (the
+ * values of the matrices do not matter); B is zeroed first. */
+void sgo_refmat_mult_adjoint_flags(int32_t nd, const int32_t* shape_y, const int32_t* shape_b,
+                                   const sgo_refmat* mats, const int32_t* refmat_of,
+                                   const uint8_t* Y, uint8_t* B);
+
+/* local_refinement_kernel (src/control_points.jl:296-311): for i < n_idx,
+ * cp[indices[i, :]..., c] = values[i, c]; indices [n_idx, nin] and values
+ * [n_idx, nout] column-major, indices 1-based; cp has shape (shape..., nout). */
+void sgo_local_refinement_overwrite(int32_t nin, int32_t nout, const int32_t* shape,
+                                    int64_t n_idx, const int32_t* indices,
+                                    const double* values, double* cp);
+
+/* local_refinement_adjoint_kernel (src/adjoint.jl:154-170): values[i, c] =
+ * cp[indices[i, :]..., c], then that entry of cp is zeroed. */
+void sgo_local_refinement_gather(int32_t nin, int32_t nout, const int32_t* shape,
+                                 int64_t n_idx, const int32_t* indices, double* values,
+                                 double* cp);
+
 int32_t sgo_max_threads(void);
 
 #ifdef __cplusplus

@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libsplinegrids_hip.so")
 
 SG_OK, SG_ERR_INVALID, SG_ERR_HIP, SG_ERR_UNSUPPORTED, SG_ERR_NO_MEMORY = 0, 1, 2, 3, 4
-SG_F32, SG_F64 = 0, 1
+SG_F32, SG_F64, SG_U8 = 0, 1, 2
 TIERS = {"native": 0, "split_f16": 1, "bf16": 2}
 MAX_NIN = 8
 MAX_FIELDS = 4
@@ -26,6 +26,9 @@ EXPORTS = (
     "sg_batch_evaluate_each", "sg_batch_autotune", "sg_batch_info", "sg_batch_describe",
     "sg_rational_workspace_bytes", "sg_evaluate_rational_fields",
     "sg_rational_adjoint_workspace_bytes", "sg_evaluate_rational_adjoint",
+    "sg_refinement_insert_knot", "sg_refinement_product_rows", "sg_refinement_product_values",
+    "sg_refinement_validate", "sg_refinement_mult", "sg_refinement_mult_adjoint",
+    "sg_local_refinement_overwrite", "sg_local_refinement_gather",
 )
 
 
@@ -45,6 +48,18 @@ class DimDesc(ctypes.Structure):
         ("max_derivative_order", ctypes.c_int32),
         ("knots_all", ctypes.POINTER(ctypes.c_double)),
         ("sample_points", ctypes.POINTER(ctypes.c_double)),
+    ]
+
+
+class RefmatDesc(ctypes.Structure):
+    """sg_refmat: a refinement matrix's sizes and device arrays."""
+    _fields_ = [
+        ("m", ctypes.c_int32),
+        ("n", ctypes.c_int32),
+        ("nnz", ctypes.c_int32),
+        ("row_pointer", ctypes.c_void_p),
+        ("column_start", ctypes.c_void_p),
+        ("nzval", ctypes.c_void_p),
     ]
 
 
@@ -99,6 +114,23 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
                                                        vp, i64, i32, vp]),
         "sg_rational_adjoint_workspace_bytes": (ctypes.c_int, [vp, i32, ctypes.POINTER(i64)]),
         "sg_evaluate_rational_adjoint": (ctypes.c_int, [vp, vp, vp, vp, vp, i64, i32, i32, vp]),
+        "sg_refinement_insert_knot": (ctypes.c_int, [vp, i32, i32, i32, ctypes.c_double, vp, vp,
+                                                     vp, i32, vp]),
+        "sg_refinement_product_rows": (ctypes.c_int, [ctypes.POINTER(RefmatDesc),
+                                                      ctypes.POINTER(RefmatDesc), vp, vp, vp]),
+        "sg_refinement_product_values": (ctypes.c_int, [ctypes.POINTER(RefmatDesc),
+                                                        ctypes.POINTER(RefmatDesc),
+                                                        ctypes.POINTER(RefmatDesc), i32, vp]),
+        "sg_refinement_validate": (ctypes.c_int, [ctypes.POINTER(RefmatDesc), vp, vp]),
+        "sg_refinement_mult": (ctypes.c_int, [i32, i32p, i32p, i32, ctypes.POINTER(RefmatDesc),
+                                              i32p, vp, vp, i32, vp]),
+        "sg_refinement_mult_adjoint": (ctypes.c_int, [i32, i32p, i32p, i32,
+                                                      ctypes.POINTER(RefmatDesc), i32p, vp, vp,
+                                                      i32, i32, vp]),
+        "sg_local_refinement_overwrite": (ctypes.c_int, [i32, i32, i32p, i64, vp, vp, vp, i32,
+                                                         vp]),
+        "sg_local_refinement_gather": (ctypes.c_int, [i32, i32, i32p, i64, vp, vp, vp, i32, i32,
+                                                       vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

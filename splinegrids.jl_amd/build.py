@@ -17,7 +17,8 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libsplinegrids_hip.so")
 SOURCES = ["sg_basis.hip", "sg_forward.hip", "sg_mfma.hip", "sg_adjoint.hip", "sg_adjoint_mma.hip",
-           "sg_batch.hip", "sg_rational.hip", "sg_api.cpp"]
+           "sg_batch.hip", "sg_rational.hip", "sg_refine.hip",
+           "sg_api.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
@@ -25,6 +26,8 @@ FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
     # packed FP32 instructions stay off (v_pk_* forms are not trusted here)
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # MFMA accumulators in ordinary VGPRs: no v_accvgpr moves around each tile
+    "-mllvm", "-amdgpu-mfma-vgpr-form",
     "-Wall", "-Wno-unused-command-line-argument",
 ]
 

@@ -14,7 +14,9 @@ import numpy as np
 import torch
 
 from . import _lib
-from .control_points import DefaultControlPoints, obtain, set_unit_cp_grid_
+from .control_points import (DefaultControlPoints, LocallyRefinedControlPoints,
+                             evaluate_adjoint_control_points_, evaluate_control_points_,
+                             obtain, set_unit_cp_grid_)
 from .spline_dimension import SplineDimension, evaluate_dimension_
 from .utils import (colmajor_zeros, current_stream_handle, dtype_code,
                     is_colmajor)
@@ -102,7 +104,8 @@ class SplineGrid:
             eval = colmajor_zeros(size_eval + (nout,), tv, dev)
         elif len(args) == 3:
             control_points, weights, eval = args
-            if not isinstance(control_points, DefaultControlPoints):
+            if not isinstance(control_points, (DefaultControlPoints,
+                                               LocallyRefinedControlPoints)):
                 control_points = DefaultControlPoints(control_points)
         else:
             raise TypeError("SplineGrid(dims, Nout) or SplineGrid(dims, control_points, weights, eval)")
@@ -185,7 +188,9 @@ def evaluate_(obj, *, derivative_order=None, control_points=None, eval=None,
     """evaluate!: dispatches like the reference.
 
     - SplineDimension: recompute its basis table (src/spline_dimension.jl:231)
-    - control-point container: nothing to do (src/control_points.jl:313)
+    - DefaultControlPoints: nothing to do (src/control_points.jl:313)
+    - LocallyRefinedControlPoints: run the THB chain of refinement products
+      and overwrites (src/control_points.jl:319-349)
     - SplineGrid: eval[J, c] = sum_I prod_d B_d[J_d, I_d, o_d] (w) P[...]
       (src/spline_grid.jl:200-230); eval is overwritten.  ``tier`` picks the
       arithmetic of the contraction ("native", "split_f16", "bf16").
@@ -194,6 +199,9 @@ def evaluate_(obj, *, derivative_order=None, control_points=None, eval=None,
         evaluate_dimension_(obj)
         return None
     if isinstance(obj, DefaultControlPoints) or isinstance(obj, torch.Tensor):
+        return None
+    if isinstance(obj, LocallyRefinedControlPoints):
+        evaluate_control_points_(obj)
         return None
     grid: SplineGrid = obj
     orders = _orders(grid, derivative_order)
@@ -315,11 +323,20 @@ def evaluate_rational_adjoint_(grid: SplineGrid, *, control_points=None, eval=No
                                                   current_stream_handle(grid.device)))
 
 
-def evaluate_adjoint_(grid: SplineGrid, *, derivative_order=None, control_points=None,
+def evaluate_adjoint_(grid, *, derivative_order=None, control_points=None,
                       eval=None, tier: str = "native", accumulate: bool = False,
                       synchronize: bool = False) -> None:
     """evaluate_adjoint! (src/adjoint.jl:52-83): control_points = M^T eval,
-    overwritten (or added to with ``accumulate=True``).  B-splines only."""
+    overwritten (or added to with ``accumulate=True``).  B-splines only.
+
+    On a control-point container it is the adjoint of that container's
+    evaluate_: nothing for DefaultControlPoints, the reversed THB chain for
+    LocallyRefinedControlPoints (src/adjoint.jl:172-205)."""
+    if isinstance(grid, (DefaultControlPoints, torch.Tensor)):
+        return None
+    if isinstance(grid, LocallyRefinedControlPoints):
+        evaluate_adjoint_control_points_(grid)
+        return None
     assert not grid.is_nurbs, "Adjoint evaluation not supported for NURBS."
     orders = _orders(grid, derivative_order)
     validate_partial_derivatives(grid.spline_dimensions, orders)
