@@ -96,6 +96,23 @@ int sg_dimension_evaluate(const void* knots_all, int32_t n_knots,
                           int32_t* sample_indices, void* eval, int32_t dtype,
                           void* stream);
 
+/* sg_dimension_evaluate without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight loops that move sample points every iteration, harnesses
+ * that forbid hipMalloc).  Same arguments, kernels and results; the scratch
+ * is a fixed device array of 2^20 doubles (8 MiB) held by every GPU the
+ * library runs on until the process ends, so n_samples * (degree + 1) *
+ * (max_derivative_order + 1) must be at most 2^20 (FP64) or 2^19 (FP32), or
+ * the call fails with SG_ERR_INVALID before launching anything.  No
+ * synchronisation: stream-ordered.  Every call on one GPU shares the array:
+ * calls that overlap on a GPU (other streams or threads) race and must be
+ * serialised by the caller. */
+int sg_dimension_evaluate_noalloc(const void* knots_all, int32_t n_knots,
+                                  const void* sample_points, int32_t n_samples,
+                                  int32_t degree, int32_t max_derivative_order,
+                                  int32_t* sample_indices, void* eval, int32_t dtype,
+                                  void* stream);
+
 /* Build a plan: uploads knots and samples, computes every dimension's FP64
  * span table and basis table (with all derivative orders up to D_d) on the
  * GPU, derives the FP32 / split-FP16 / BF16 operand tables, picks the kernels

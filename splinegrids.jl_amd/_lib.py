@@ -19,7 +19,8 @@ MAX_FIELDS = 4
 
 # every symbol include/splinegrids_hip.h declares
 EXPORTS = (
-    "sg_last_error", "sg_version", "sg_dimension_evaluate", "sg_plan_create",
+    "sg_last_error", "sg_version", "sg_dimension_evaluate", "sg_dimension_evaluate_noalloc",
+    "sg_plan_create",
     "sg_plan_destroy", "sg_plan_basis", "sg_plan_device_bytes", "sg_plan_describe", "sg_evaluate",
     "sg_evaluate_fields", "sg_evaluate_adjoint", "sg_stream_synchronize",
     "sg_batch_create", "sg_batch_destroy", "sg_batch_set_arrays", "sg_batch_evaluate",
@@ -90,6 +91,8 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
         "sg_last_error": (ctypes.c_char_p, []),
         "sg_version": (ctypes.c_int, [i32p, i32p, i32p]),
         "sg_dimension_evaluate": (ctypes.c_int, [vp, i32, vp, i32, i32, i32, vp, vp, i32, vp]),
+        "sg_dimension_evaluate_noalloc": (ctypes.c_int, [vp, i32, vp, i32, i32, i32, vp, vp, i32,
+                                                         vp]),
         "sg_plan_create": (ctypes.c_int, [ctypes.POINTER(GridDesc), vp, ctypes.POINTER(vp)]),
         "sg_plan_destroy": (ctypes.c_int, [vp]),
         "sg_plan_basis": (ctypes.c_int, [vp, i32, f64p, i32p]),
