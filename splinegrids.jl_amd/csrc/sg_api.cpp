@@ -466,45 +466,19 @@ int build_plan(const sg_grid_desc* desc, hipStream_t s, bool want_adjoint, Plan*
       compute_tiling(*pl, pl->fwd, rows, wx);
     }
     pl->fwd.xcd_map = xcd;
-    // MFMA tiers: 64-sample tiles, every wave all 64 samples of dimension 1
-    // in rows of 16.  3-D tiles take four samples j3 (one per wave) over a
-    // chunk of up to 128 rows while the four planes' windows (every channel
-    // of a launch) fit three quarters of a workgroup's LDS; otherwise, and in
-    // 2-D, one plane with the waves splitting the chunk (up to 64 rows each),
-    // as long as one channel (+ weight) fits.
-    const int full_nch = std::min(pl->nout, pl->nurbs ? 3 : 4) + (pl->nurbs ? 1 : 0);
-    const int min_nch = 1 + (pl->nurbs ? 1 : 0);
-    auto mma_tiling = [&](int planes, int rows) {  // rows: a multiple of 16
-      pl->mma = Tiling();
-      pl->mma.tile_w = kLanes;
-      if (planes > 1) pl->mma.chunk_h = rows;  // every wave the whole chunk
-      compute_tiling(*pl, pl->mma, rows, 1);   // else chunk = rows per wave * 4
-      compute_ztiling(*pl, pl->mma, planes);
+    // MFMA tiers: 64-sample tiles, each wave 16 samples of dimension 1
+    // (LDS: the widest channel block, or four fields of one channel)
+    const int mnch = std::max(std::min(pl->nout, 4) + (pl->nurbs ? 1 : 0), 1);
+    auto mma_worst = [&]() {
+      return std::max(mma_lds(*pl, mnch, 1), mma_lds(*pl, 1, SG_MAX_FIELDS));
     };
-    const int env_rows = env_int("SG_MMA_ROWS", 0);
-    const int env_planes = env_int("SG_MMA_PLANES", 0);
-    bool found = false;
-    if (pl->nin == 3 && env_planes != 1) {
-      for (int rows : {128, 64, 32, 16}) {
-        if (env_rows && rows != (env_rows + 15) / 16 * 16) continue;
-        mma_tiling(mma_max_planes(3), rows);
-        if (mma_lds(*pl, full_nch, 1) <= static_cast<size_t>(kMaxLds) * 3 / 4) {
-          found = true;
-          break;
-        }
-      }
+    int mrows = 32;
+    compute_tiling(*pl, pl->mma, mrows, 1);
+    while (mma_worst() > static_cast<size_t>(kMaxLds) && mrows > 4) {
+      mrows /= 2;
+      compute_tiling(*pl, pl->mma, mrows, 1);
     }
-    for (int rows : {64, 32, 16}) {
-      if (found) break;
-      if (env_rows && rows != (env_rows + 15) / 16 * 16) continue;
-      mma_tiling(1, rows);
-      found = mma_lds(*pl, min_nch, 1) <= static_cast<size_t>(kMaxLds);
-    }
-    if (!found) {  // wide windows (few samples per span): all of a CU's LDS
-      mma_tiling(1, 16);
-      found = mma_lds(*pl, min_nch, 1) <= static_cast<size_t>(kMaxLdsHw);
-    }
-    if (!found) pl->mma = Tiling();
+    if (mma_worst() > static_cast<size_t>(kMaxLds)) pl->mma = Tiling();
     pl->mma.xcd_map = xcd;
     ok = forward_lds_worst(*pl, pl->fwd) <= static_cast<size_t>(kMaxLds);
   }

@@ -571,7 +571,7 @@ void pack_queues(const std::vector<std::vector<BTile>>& grid_tiles,
 // when a grid cannot take the tiers.  A workgroup reserves at most the LDS
 // that lets kMmaBatchPerCu of them share a CU; grids whose windows need more
 // are evaluated with their own tier launches after the persistent one.
-constexpr int kMmaBatchPerCu = 2;
+constexpr int kMmaBatchPerCu = 4;
 
 int build_mma_tiles(sg_batch* b) {
   b->mma_ok = false;
@@ -586,9 +586,8 @@ int build_mma_tiles(sg_batch* b) {
   for (size_t gi = 0; gi < b->plans.size(); ++gi) {
     const Plan& pl = *b->plans[gi];
     size_t need = 0;
-    const int ncmax = mma_channels(pl, 1);
     for (int c0 = 0; c0 < pl.nout;) {
-      const int nc = std::min(pl.nout - c0, ncmax);
+      const int nc = std::min(pl.nout - c0, pl.nurbs ? 3 : 4);
       need = std::max(need, mma_lds(pl, nc + (pl.nurbs ? 1 : 0), 1));
       c0 += nc;
     }
@@ -599,23 +598,22 @@ int build_mma_tiles(sg_batch* b) {
     const Tiling& t = pl.mma;
     const int m1 = pl.dims[0].m, m2 = pl.dims[1].m;
     const int m3 = pl.nin == 3 ? pl.dims[2].m : 1;
-    const int planes = pl.nin == 3 ? pl.mma.j3_per_block : 1;  // samples j3 per tile
     const int chunk = t.chunk_rows();
     std::vector<BTile> tiles;
     long long cost = 0;
     for (int c0 = 0; c0 < pl.nout;) {
-      const int nc = std::min(pl.nout - c0, ncmax);
+      const int nc = std::min(pl.nout - c0, pl.nurbs ? 3 : 4);
       const int r = static_cast<int>(b->mrec_grid.size());
       b->mrec_grid.push_back(static_cast<int>(gi));
       b->mrec_c0.push_back(c0);
       b->mrec_nc.push_back(nc);
       lds = std::max(lds, mma_lds(pl, nc + (pl.nurbs ? 1 : 0), 1));
-      for (int z = 0; z * planes < m3; ++z)
+      for (int z = 0; z < m3; ++z)
         for (int y = 0; y < t.n_chunks2; ++y)
           for (int x = 0; x < t.n_tiles1; ++x) {
             tiles.push_back(BTile{r, x, y, z});
             cost += static_cast<long long>(std::min(kLanes, m1 - x * kLanes)) *
-                    std::min(chunk, m2 - y * chunk) * std::min(planes, m3 - z * planes) * nc;
+                    std::min(chunk, m2 - y * chunk) * nc;
           }
       c0 += nc;
     }

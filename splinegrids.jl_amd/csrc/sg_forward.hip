@@ -83,7 +83,7 @@ __global__ __launch_bounds__(kThreads) void fwd_stream(
   const int cs = W1 * W2;  // LDS elements per channel
 
   // ---- stage 1 ----
-  stage1_planes<T, NIN, NC, NURBS, 1>(G, a.src, j3, 1, lo1, W1, lo2, W2);
+  stage1_band_window<T, NIN, NC, NURBS>(G, a.src, j3, lo1, W1, lo2, W2);
   __syncthreads();
 
   // ---- stage 2 ----

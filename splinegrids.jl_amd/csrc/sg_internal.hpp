@@ -19,11 +19,7 @@ constexpr int kLanes = 64;
 constexpr int kWaves = 4;               // waves per workgroup
 constexpr int kThreads = kLanes * kWaves;
 constexpr int kMaxStreamDegree = 5;     // streaming kernels: p1, p2 <= 5
-constexpr int kMaxLds = 64 * 1024;      // LDS bytes a workgroup claims by default
-// The most LDS one workgroup may claim on gfx950 (hipDeviceAttribute-
-// MaxSharedMemoryPerBlock = 163840): tilings whose windows do not fit
-// kMaxLds fall back to it, at one workgroup per CU.
-constexpr int kMaxLdsHw = 160 * 1024;
+constexpr int kMaxLds = 64 * 1024;      // LDS bytes one workgroup may claim
 constexpr int kAdjMmaWin = 16;         // matrix-core adjoint: window edge (one MFMA tile)
 
 // Workgroup copy of n >= 1 elements of NF sources into LDS:
@@ -216,12 +212,6 @@ hipError_t launch_forward(const Plan& plan, int nf, const int32_t* orders,
 
 // ---- sg_mfma.hip ----
 bool mma_supported(const Plan& plan);
-// most samples j3 per tile of the MFMA kernels (3-D: one per wave; 2-D: 1);
-// a plan's MFMA tiling takes 4 or 1 (Tiling::j3_per_block)
-int mma_max_planes(int nin);
-// value channels per MFMA launch of nf fields (the most whose LDS fits)
-int mma_channels(const Plan& plan, int nf);
-int mma_fields(const Plan& plan);
 // LDS bytes of an MFMA forward launch: nch window channels, nf fields.
 size_t mma_lds(const Plan& plan, int nch, int nf);
 hipError_t launch_forward_mma(const Plan& plan, int tier, int nf, const int32_t* orders,

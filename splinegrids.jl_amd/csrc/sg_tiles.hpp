@@ -19,76 +19,54 @@ struct BandSrc {
 };
 
 // Stage 1: the band window of control points of a tile, contracted along
-// dimension 3 for the tile's samples j3_0 .. j3_0 + nplanes - 1 (PL at most):
-//   G[w][c][q2][q1] = sum_d B3[j3_0 + w, d] * (w *) P[lo1+q1, lo2+q2, s3-p3+d, c0+c]
-//   G[w][NC][q2][q1] = sum_d B3[j3_0 + w, d] * w[...]                    (NURBS)
-// for q1 < W1, q2 < W2 (2-D: one plane, no dimension-3 sum).  Each
-// control-point plane k3 in the union of the planes' bands is loaded once
-// and serves every plane whose band holds it; each plane sums its terms in
-// increasing k3 (the order of d), as a single-plane contraction would.  All
-// threads of the block take part; the caller synchronises before reading G.
-template <typename T, int NIN, int NC, bool NURBS, int PL>
-__device__ inline void stage1_planes(T* G, const BandSrc<T>& b, int j3_0, int nplanes, int lo1,
-                                     int W1, int lo2, int W2) {
-  constexpr int NCH = NC + (NURBS ? 1 : 0);
+// dimension 3 for sample j3:
+//   G[c][q2][q1] = sum_d B3[j3, d] * (w *) P[lo1+q1, lo2+q2, s3-p3+d, c0+c]
+//   G[NC][q2][q1] = sum_d B3[j3, d] * w[...]                    (NURBS)
+// for q1 < W1, q2 < W2 (2-D: no dimension-3 sum).  All threads of the block
+// take part; the caller synchronises before reading G.
+template <typename T, int NIN, int NC, bool NURBS>
+__device__ inline void stage1_band_window(T* G, const BandSrc<T>& b, int j3, int lo1,
+                                          int W1, int lo2, int W2) {
   const int cs = W1 * W2;
-  int s3[PL], lo3 = 0, hi3 = 0;
-  if constexpr (NIN == 3) {
-#pragma unroll
-    for (int w = 0; w < PL; ++w) s3[w] = b.span3[j3_0 + min(w, nplanes - 1)];
-    lo3 = s3[0] - b.p3;
-    hi3 = s3[PL - 1];
-  }
   for (int idx = threadIdx.x; idx < cs; idx += blockDim.x) {
     const int q2 = idx / W1, q1 = idx - q2 * W1;
     const long long cpi =
         static_cast<long long>(lo1 + q1) + static_cast<long long>(b.n1) * (lo2 + q2);
-    T acc[PL][NCH];
+    T acc[NC];
+    T accw = T(0);
 #pragma unroll
-    for (int w = 0; w < PL; ++w)
-#pragma unroll
-      for (int c = 0; c < NCH; ++c) acc[w][c] = T(0);
+    for (int c = 0; c < NC; ++c) acc[c] = T(0);
     if constexpr (NIN == 2) {
       if constexpr (NURBS) {
-        const T wt = b.W[cpi];
-        acc[0][NC] = wt;
+        const T w = b.W[cpi];
+        accw = w;
 #pragma unroll
-        for (int c = 0; c < NC; ++c) acc[0][c] = wt * b.P[cpi + b.ncp * (b.c0 + c)];
+        for (int c = 0; c < NC; ++c) acc[c] = w * b.P[cpi + b.ncp * (b.c0 + c)];
       } else {
 #pragma unroll
-        for (int c = 0; c < NC; ++c) acc[0][c] = b.P[cpi + b.ncp * (b.c0 + c)];
+        for (int c = 0; c < NC; ++c) acc[c] = b.P[cpi + b.ncp * (b.c0 + c)];
       }
     } else {
+      const int s3 = b.span3[j3];
+      const T* b3 = b.tab3 + static_cast<size_t>(j3) * (b.p3 + 1);
       const long long plane = static_cast<long long>(b.n1) * b.n2;
-      for (int k3 = lo3; k3 <= hi3; ++k3) {
-        const long long off = cpi + plane * k3;
-        T v[NC];
+      for (int d = 0; d <= b.p3; ++d) {
+        const long long off = cpi + plane * (s3 - b.p3 + d);
+        const T bd = b3[d];
+        if constexpr (NURBS) {
+          const T bw = bd * b.W[off];
+          accw += bw;
 #pragma unroll
-        for (int c = 0; c < NC; ++c) v[c] = b.P[off + b.ncp * (b.c0 + c)];
-        const T wk = NURBS ? b.W[off] : T(1);
+          for (int c = 0; c < NC; ++c) acc[c] += bw * b.P[off + b.ncp * (b.c0 + c)];
+        } else {
 #pragma unroll
-        for (int w = 0; w < PL; ++w) {
-          const int d = k3 - (s3[w] - b.p3);
-          if (w < nplanes && d >= 0 && d <= b.p3) {
-            const T bd = b.tab3[static_cast<size_t>(j3_0 + w) * (b.p3 + 1) + d];
-            if constexpr (NURBS) {
-              const T bw = bd * wk;
-              acc[w][NC] += bw;
-#pragma unroll
-              for (int c = 0; c < NC; ++c) acc[w][c] += bw * v[c];
-            } else {
-#pragma unroll
-              for (int c = 0; c < NC; ++c) acc[w][c] += bd * v[c];
-            }
-          }
+          for (int c = 0; c < NC; ++c) acc[c] += bd * b.P[off + b.ncp * (b.c0 + c)];
         }
       }
     }
 #pragma unroll
-    for (int w = 0; w < PL; ++w)
-      if (w < nplanes)  // the LDS holds nplanes planes
-#pragma unroll
-        for (int c = 0; c < NCH; ++c) G[(w * NCH + c) * cs + idx] = acc[w][c];
+    for (int c = 0; c < NC; ++c) G[c * cs + idx] = acc[c];
+    if constexpr (NURBS) G[NC * cs + idx] = accw;
   }
 }
 
