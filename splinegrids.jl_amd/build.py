@@ -17,7 +17,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libsplinegrids_hip.so")
 SOURCES = ["sg_basis.hip", "sg_forward.hip", "sg_mfma.hip", "sg_adjoint.hip", "sg_adjoint_mma.hip",
-           "sg_batch.hip", "sg_rational.hip", "sg_noalloc.hip", "sg_refine.hip",
+           "sg_batch.hip", "sg_rational.hip", "sg_noalloc.hip", "sg_refine.hip", "sg_plan_host.hip",
            "sg_api.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

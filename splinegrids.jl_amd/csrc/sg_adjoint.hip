@@ -24,7 +24,7 @@
 //               (k1, k2, k3).
 // General path (any Nin, any degree, any sample order): adj_csr, one thread
 // per control point, walking each dimension's transposed band list.
-#include "sg_internal.hpp"
+#include "sg_plan_host.hpp"
 
 namespace sg {
 
@@ -457,17 +457,6 @@ AdjKernel<T> pick_adj(int p2, int nc) {
 
 }  // namespace
 
-size_t adjoint_stream_lds(const Plan& pl, int nc, size_t elem) {
-  const Tiling& t = pl.adj;
-  const int p3 = pl.nin == 3 ? pl.dims[2].p : 0;
-  const size_t ls = kLanes + 16 / elem;
-  const size_t chunk = static_cast<size_t>(t.chunk_rows());
-  return static_cast<size_t>(kWaves) * t.w2wave_max * nc * ls * elem +
-         static_cast<size_t>(t.w1max) * ls * elem +
-         static_cast<size_t>(p3 + 1) * t.w1max * t.w2max * nc * elem +
-         chunk * (pl.dims[1].p + 1) * elem +
-         (chunk + 2 * kWaves + 2 * t.w1max) * sizeof(int);
-}
 
 namespace {
 

@@ -19,7 +19,7 @@
 #include <sstream>
 #include <vector>
 
-#include "sg_queue.hpp"
+#include "sg_plan_host.hpp"
 #include "sg_tiles.hpp"
 
 namespace sg {
@@ -45,7 +45,6 @@ struct BGrid {
 
 namespace {
 
-constexpr int kBC = 4;  // value channels per pass; the weight is slab 4
 
 
 // Everything a channel-block pass of a tile needs, set up once per tile.
@@ -76,12 +75,7 @@ struct TileCtx {
 // {w[0..3], k3}, where w[pz] is B3 of plane pz at k3 (0 outside its band).
 // Every k3 is loaded once and serves all planes.  An extra entry (zero
 // weights, a valid k3) lets stage 1 take the entries in pairs.
-constexpr int kEntry = 8;  // floats per entry: weights (16-byte aligned), k3, padding
-constexpr int kEntryK = 4;
-
-__host__ __device__ constexpr int plane_list_floats(int p3) {
-  return (kWaves * (p3 + 1) + 1) * kEntry;
-}
+// (kEntry floats per entry: weights (16-byte aligned), k3 at kEntryK, padding)
 
 // Builds the list into `list` (16-byte aligned LDS) and returns its length.
 // The spans of the planes are non-decreasing, so band q adds the planes
@@ -501,70 +495,7 @@ void free_batch(sg_batch* b) {
   delete b;
 }
 
-// Window sizes of a grid's tiles for a rows-per-wave choice.
-struct GridTiling {
-  int rows, ntx, nty, w1, w2;
-  size_t lds;  // bytes of LDS one tile of this grid needs
-};
 
-GridTiling grid_tiling(const Plan& pl, int rows) {
-  const DimDev& d1 = pl.dims[0];
-  const DimDev& d2 = pl.dims[1];
-  GridTiling t{rows, (d1.m + kLanes - 1) / kLanes, 0, 0, 0, 0};
-  const int chunk = rows * kWaves;
-  t.nty = (d2.m + chunk - 1) / chunk;
-  for (int i = 0; i < t.ntx; ++i) {
-    const int a = i * kLanes, e = std::min(a + kLanes, d1.m) - 1;
-    t.w1 = std::max(t.w1, d1.h_span[e] - d1.h_span[a] + d1.p + 1);
-  }
-  for (int c = 0; c < t.nty; ++c) {
-    const int a = c * chunk, e = std::min(a + chunk, d2.m) - 1;
-    t.w2 = std::max(t.w2, d2.h_span[e] - d2.h_span[a] + d2.p + 1);
-  }
-  const size_t nch = std::min(pl.nout, kBC) + (pl.nurbs ? 1 : 0);
-  const size_t planes = pl.nin == 3 ? kWaves : 1;
-  // window slabs, alignment and the plane list of 3-D tiles
-  const size_t list = pl.nin == 3 ? 3 + plane_list_floats(pl.dims[2].p) : 0;
-  t.lds = (planes * nch * t.w1 * t.w2 + list) * sizeof(float);
-  return t;
-}
-
-// LDS of one CU; a persistent launch of `bpc` workgroups per CU gives each
-// at most a bpc-th of it, or fewer workgroups than asked are resident.
-constexpr size_t kLdsPerCu = 160 * 1024;
-
-static size_t lds_cap(int blocks_per_cu) {
-  const size_t share = kLdsPerCu / static_cast<size_t>(std::max(1, blocks_per_cu));
-  return std::min(static_cast<size_t>(kMaxLds), (share & ~size_t(511)) - 512);
-}
-
-// The queues of a tile list: grids go whole to the least loaded queue,
-// largest first; a queue holds its grids in that order.  Writes the tiles in
-// queue order and the queue starts (host_q: zero counters, then starts).
-void pack_queues(const std::vector<std::vector<BTile>>& grid_tiles,
-                 const std::vector<long long>& grid_cost, std::vector<BTile>& sorted,
-                 int* host_q) {
-  std::vector<size_t> order(grid_tiles.size());
-  std::iota(order.begin(), order.end(), 0);
-  std::stable_sort(order.begin(), order.end(),
-                   [&](size_t a, size_t c) { return grid_cost[a] > grid_cost[c]; });
-  std::vector<std::vector<size_t>> queue_grids(kXcds);
-  long long load[kXcds] = {};
-  for (size_t g : order) {
-    const int q = static_cast<int>(std::min_element(load, load + kXcds) - load);
-    load[q] += grid_cost[g];
-    queue_grids[q].push_back(g);
-  }
-  sorted.clear();
-  std::fill(host_q, host_q + kQueueInts, 0);
-  int* start = host_q + kXcds * kQueueStride;
-  for (int q = 0; q < kXcds; ++q) {
-    start[q] = static_cast<int>(sorted.size());
-    for (size_t g : queue_grids[q])
-      sorted.insert(sorted.end(), grid_tiles[g].begin(), grid_tiles[g].end());
-  }
-  start[kXcds] = static_cast<int>(sorted.size());
-}
 
 // MFMA tiers: argument records and tiles of every grid (the grids' arrays
 // are filled into the records by sg_batch_set_arrays).  Leaves mma_ok false

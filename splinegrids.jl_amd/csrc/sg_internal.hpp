@@ -212,8 +212,6 @@ hipError_t launch_forward(const Plan& plan, int nf, const int32_t* orders,
 
 // ---- sg_mfma.hip ----
 bool mma_supported(const Plan& plan);
-// LDS bytes of an MFMA forward launch: nch window channels, nf fields.
-size_t mma_lds(const Plan& plan, int nch, int nf);
 hipError_t launch_forward_mma(const Plan& plan, int tier, int nf, const int32_t* orders,
                               const float* P, const float* W, float* const* outs,
                               hipStream_t s);
@@ -246,7 +244,6 @@ hipError_t launch_adjoint(const Plan& plan, const int32_t* orders,
                           const void* ebar, void* Pbar, int dtype,
                           int accumulate, hipStream_t s);
 size_t adjoint_work_bytes(const Plan& plan);
-size_t adjoint_stream_lds(const Plan& plan, int nc, size_t elem);
 // Sum the partial windows [unit][w3max][nout][w2][w1] of tiling tl into
 // Pbar (overwrite or accumulate), one thread per control point.
 hipError_t launch_adjoint_gather(const Plan& plan, const Tiling& tl,

@@ -21,7 +21,7 @@
 #include <vector>
 
 #include "sg_mma_ops.hpp"
-#include "sg_queue.hpp"
+#include "sg_plan_host.hpp"
 #include "sg_tiles.hpp"
 
 namespace sg {
@@ -321,15 +321,6 @@ MmaKernel pick_mma(int tier, int nin, int nf, int nc, bool nurbs) {
 }  // namespace
 
 bool mma_supported(const Plan& pl) { return pl.stream_ok && pl.mma.n_tiles1 > 0; }
-
-// LDS of mma_stream: G for nch channels, the chunk's B2 rows of nf fields and
-// its spans.
-size_t mma_lds(const Plan& pl, int nch, int nf) {
-  const Tiling& t = pl.mma;
-  return (static_cast<size_t>(nch) * t.w1max * t.w2max +
-          static_cast<size_t>(nf) * t.chunk_rows() * (pl.dims[1].p + 1)) * sizeof(float) +
-         static_cast<size_t>(t.chunk_rows()) * sizeof(int);
-}
 
 hipError_t launch_forward_mma(const Plan& pl, int tier, int nf, const int32_t* orders,
                               const float* P, const float* W, float* const* outs,

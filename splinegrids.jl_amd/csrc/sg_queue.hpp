@@ -23,7 +23,7 @@ constexpr int kQueueInts = kXcds * kQueueStride + kXcds + 1;
 
 namespace {
 
-__global__ void queue_reset(int* counters) {
+[[maybe_unused]] __global__ void queue_reset(int* counters) {
   if (blockIdx.x == 0 && threadIdx.x < kXcds) counters[threadIdx.x * kQueueStride] = 0;
 }
 
