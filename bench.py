@@ -10,7 +10,9 @@ max and RMS error against this library's FP64 path.  value = sample-points/s
 = samples / time.  Inputs are resident in HBM before timing starts.
 
 Prints one JSON line.  Other workloads (--workload c1|c2|c3n|c3d|c3a|c4|c4a|c5)
-are for profiling and parity runs.
+are for profiling and parity runs; copy|fill calibrate the HBM counters and
+ceiling prints the write bandwidth of the forward's bytes as a plain fill and
+in the forward kernel's store pattern.
 """
 from __future__ import annotations
 
@@ -354,6 +356,24 @@ def traffic_from_profiles(workload_name: str):
 # main
 # --------------------------------------------------------------------------
 
+def store_ceiling():
+    """Write bandwidth of the 805 MB of the 512 x 512 x 256 x 3 forward as a
+    plain fill and in the forward kernel's store pattern with nothing
+    computed (splinegrids.jl_amd/csrc/store_ceiling.hip, built by build()):
+    the ceilings its roofline fraction is read against."""
+    import subprocess
+
+    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "splinegrids.jl_amd",
+                       "store_ceiling")
+    if not os.path.exists(exe):
+        print("store_ceiling is not built: run __graft_entry__.build()", file=sys.stderr)
+        return 1
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    sys.stdout.write(r.stdout)
+    sys.stderr.write(r.stderr)
+    return r.returncode
+
+
 def copy_calibration(args, wl):
     """Time torch's device copy of 1 GiB; its counters calibrate FETCH/WRITE_SIZE."""
     import torch
@@ -529,6 +549,8 @@ def main(argv=None):
 
     import __graft_entry__
 
+    if args.workload == "ceiling":
+        return store_ceiling()
     sg = __graft_entry__.package()
     wl = workload(args.workload)
     if wl.get("copy"):

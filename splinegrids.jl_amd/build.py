@@ -16,6 +16,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libsplinegrids_hip.so")
+# the write-bandwidth probe bench.py --workload ceiling runs (not in the library)
+CEILING = os.path.join(HERE, "store_ceiling")
 SOURCES = ["sg_basis.hip", "sg_forward.hip", "sg_mfma.hip", "sg_adjoint.hip", "sg_adjoint_mma.hip",
            "sg_batch.hip", "sg_rational.hip", "sg_noalloc.hip", "sg_refine.hip", "sg_plan_host.hip",
            "sg_api.cpp"]
@@ -60,6 +62,12 @@ def build(verbose: bool = False, jobs: int | None = None) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
     os.replace(tmp, LIB)
+    cmd = [HIPCC, *FLAGS, "-x", "hip", os.path.join(CSRC, "store_ceiling.hip"), "-o",
+           CEILING + ".tmp"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed for store_ceiling.hip:\n{r.stderr}")
+    os.replace(CEILING + ".tmp", CEILING)
     return LIB
 
 
