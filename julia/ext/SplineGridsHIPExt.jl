@@ -19,6 +19,15 @@
 #                                                                    src/utils.jl:19-29)
 #   evaluate!(::SplineGrid; ...)        -> sg_evaluate             (src/spline_grid.jl:200-230)
 #   evaluate_adjoint!(::SplineGrid; ...) -> sg_evaluate_adjoint    (src/adjoint.jl:52-83)
+#   RefinementMatrix * RefinementMatrix  -> sg_refinement_product_rows / _values
+#                                                                   (src/refinement_matrix.jl:273-327)
+#   mult!(Y, As, B, dims)               -> sg_refinement_mult      (src/refinement_matrix.jl:421-445)
+#   mult_adjoint!(B, As, Y, dims)       -> sg_refinement_mult_adjoint (src/adjoint.jl:127-152;
+#                                          no atomics; Flag arrays take the structural form)
+# for refinement matrices and arrays on ROCArrays; evaluate! and
+# evaluate_adjoint! of LocallyRefinedControlPoints then run on these
+# (their overwrite / gather kernels are the reference's own KernelAbstractions
+# kernels, which run on the AMDGPU backend as they are).
 # Validation, size asserts and their messages stay the reference's
 # (validate_partial_derivatives, the @asserts), before any ccall.
 module SplineGridsHIPExt
@@ -26,7 +35,7 @@ module SplineGridsHIPExt
 using SplineGrids
 using SplineGrids: AbstractControlPointArray, obtain, is_nurbs,
                    validate_partial_derivatives, hip_evaluate!, hip_evaluate_adjoint!,
-                   spline_eval
+                   spline_eval, RefinementMatrix, Flag, validate_mult_input
 using AMDGPU
 using Libdl
 
@@ -38,7 +47,9 @@ const SYMBOLS = (:sg_last_error, :sg_dimension_evaluate, :sg_plan_create, :sg_pl
     :sg_evaluate, :sg_evaluate_fields, :sg_evaluate_adjoint,
     :sg_rational_adjoint_workspace_bytes, :sg_evaluate_rational_adjoint,
     :sg_rational_workspace_bytes, :sg_evaluate_rational_fields, :sg_batch_create,
-    :sg_batch_destroy, :sg_batch_set_arrays, :sg_batch_evaluate, :sg_batch_autotune)
+    :sg_batch_destroy, :sg_batch_set_arrays, :sg_batch_evaluate, :sg_batch_autotune,
+    :sg_refinement_product_rows, :sg_refinement_product_values, :sg_refinement_mult,
+    :sg_refinement_mult_adjoint)
 
 # Function pointers, resolved once in __init__ and only read afterwards (so
 # finalizers may use them without taking a lock).
@@ -359,3 +370,90 @@ function autotune!(b::Batch; reps::Integer = 3)
 end
 
 end # module SplineGridsHIPExt
+
+# ---------------------------------------------------------------------------
+# refinement matrices and THB levels (src/refinement_matrix.jl, src/adjoint.jl:85-152)
+# ---------------------------------------------------------------------------
+
+# sg_refmat (same field order, sizes and alignment)
+struct SgRefmat
+    m::Int32
+    n::Int32
+    nnz::Int32
+    row_pointer::Ptr{Int32}
+    column_start::Ptr{Int32}
+    nzval::Ptr{Cvoid}
+end
+
+const HIPRefinementMatrix{Tv} =
+    RefinementMatrix{Tv, Int32, <:ROCVector{Int32}, <:ROCVector{Tv}} where {Tv}
+
+sg_refmat(A) = SgRefmat(Int32(A.m), Int32(A.n), Int32(length(A.nzval)),
+    reinterpret(Ptr{Int32}, pointer(A.row_pointer)),
+    reinterpret(Ptr{Int32}, pointer(A.column_start)), devptr(A.nzval))
+
+function Base.:*(A::HIPRefinementMatrix{Tv}, B::HIPRefinementMatrix{Tv}) where {Tv}
+    A.n == B.m || throw(DimensionMismatch("Inner dimensions must match"))
+    n_nonzero = ROCVector{Int32}(undef, A.m)
+    column_start = ROCVector{Int32}(undef, A.m)
+    dA, dB = Ref(sg_refmat(A)), Ref(sg_refmat(B))
+    GC.@preserve A B n_nonzero column_start begin
+        check(ccall(FN[:sg_refinement_product_rows], Cint,
+            (Ptr{SgRefmat}, Ptr{SgRefmat}, Ptr{Int32}, Ptr{Int32}, Ptr{Cvoid}),
+            dA, dB, reinterpret(Ptr{Int32}, pointer(n_nonzero)),
+            reinterpret(Ptr{Int32}, pointer(column_start)), stream_handle()))
+    end
+    row_pointer = similar(n_nonzero)
+    row_pointer[1:1] .= 1
+    A.m > 1 && (row_pointer[2:end] .= 1 .+ cumsum(n_nonzero[1:(end - 1)]))
+    nzval = AMDGPU.zeros(Tv, Int(sum(n_nonzero)))
+    C = (m = A.m, n = B.n, row_pointer = row_pointer, column_start = column_start,
+        nzval = nzval)
+    dC = Ref(sg_refmat(C))
+    GC.@preserve A B C begin
+        check(ccall(FN[:sg_refinement_product_values], Cint,
+            (Ptr{SgRefmat}, Ptr{SgRefmat}, Ptr{SgRefmat}, Int32, Ptr{Cvoid}),
+            dA, dB, dC, dtype_code(Tv), stream_handle()))
+    end
+    # the reference's constructor validates the rows (src/refinement_matrix.jl:30-49)
+    return RefinementMatrix(A.m, B.n, row_pointer, column_start, nzval)
+end
+
+function mult_args(Y, As, B, dims_refinement)
+    shape_y = Int32[size(Y)...]
+    shape_b = Int32[size(B)...]
+    mats = SgRefmat[sg_refmat(A) for A in As]
+    dims = Int32[dims_refinement...]
+    return shape_y, shape_b, mats, dims
+end
+
+function SplineGrids.mult!(Y::ROCArray{Tv}, As::NTuple{N, HIPRefinementMatrix{Tv}},
+        B::ROCArray{Tv}, dims_refinement::NTuple{N, <:Integer})::Nothing where {N, Tv}
+    validate_mult_input(Y, As, B, dims_refinement)
+    shape_y, shape_b, mats, dims = mult_args(Y, As, B, dims_refinement)
+    GC.@preserve As Y B shape_y shape_b mats dims begin
+        check(ccall(FN[:sg_refinement_mult], Cint,
+            (Int32, Ptr{Int32}, Ptr{Int32}, Int32, Ptr{SgRefmat}, Ptr{Int32}, Ptr{Cvoid},
+                Ptr{Cvoid}, Int32, Ptr{Cvoid}),
+            Int32(ndims(Y)), shape_y, shape_b, Int32(N), mats, dims, devptr(B), devptr(Y),
+            dtype_code(Tv), stream_handle()))
+    end
+    return nothing
+end
+
+# B = M^T Y, overwritten as in the reference (src/adjoint.jl:135); Flag arrays
+# (deactivate_overwritten_control_points!) take the structural uint8 form
+function SplineGrids.mult_adjoint!(B::ROCArray{T}, As::NTuple{N, <:HIPRefinementMatrix},
+        Y::ROCArray{T}, dims_refinement::NTuple{N, <:Integer})::Nothing where {N, T}
+    validate_mult_input(Y, As, B, dims_refinement)
+    code = T === Flag ? Int32(2) : dtype_code(T)
+    shape_y, shape_b, mats, dims = mult_args(Y, As, B, dims_refinement)
+    GC.@preserve As Y B shape_y shape_b mats dims begin
+        check(ccall(FN[:sg_refinement_mult_adjoint], Cint,
+            (Int32, Ptr{Int32}, Ptr{Int32}, Int32, Ptr{SgRefmat}, Ptr{Int32}, Ptr{Cvoid},
+                Ptr{Cvoid}, Int32, Int32, Ptr{Cvoid}),
+            Int32(ndims(Y)), shape_y, shape_b, Int32(N), mats, dims, devptr(Y), devptr(B),
+            code, Int32(0), stream_handle()))
+    end
+    return nothing
+end

@@ -26,7 +26,7 @@ C_TO_JULIA = {
     "sg_batch*": {"Ptr{Cvoid}"}, "const sg_batch*": {"Ptr{Cvoid}"},
     "sg_batch**": {"Ptr{Ptr{Cvoid}}"}, "void* const*": {"Ptr{Ptr{Cvoid}}"},
     "const void* const*": {"Ptr{Ptr{Cvoid}}"}, "char*": {"Ptr{UInt8}"},
-    "const char*": {"Cstring"},
+    "const char*": {"Cstring"}, "const sg_refmat*": {"Ptr{SgRefmat}"},
 }
 
 
@@ -118,6 +118,11 @@ def test_descriptor_structs_follow_the_header():
                      r"const double\* sample_points;", h, re.S)
     assert re.search(r"int32_t n_in;.*?int32_t n_out;.*?int32_t is_nurbs;.*?"
                      r"int32_t device;.*?const sg_dim_desc\* dims;", h, re.S)
+    refmat = re.search(r"struct SgRefmat(.*?)end", text, re.S).group(1).split()
+    assert refmat == ["m::Int32", "n::Int32", "nnz::Int32", "row_pointer::Ptr{Int32}",
+                      "column_start::Ptr{Int32}", "nzval::Ptr{Cvoid}"]
+    assert re.search(r"int32_t m, n, nnz;\s*const int32_t\* row_pointer;\s*"
+                     r"const int32_t\* column_start;\s*const void\* nzval;", h, re.S)
 
 
 def test_project_fragment_lists_every_extension():
