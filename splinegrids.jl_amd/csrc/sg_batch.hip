@@ -60,6 +60,7 @@ struct TileCtx {
   const float* list;  // 3-D tiles: the plane list (build_plane_list)
   int nk;             // ... and its length
   int n1, p3, nch, cs, W1, lo1, lo2, rb, re, j3_0, j3, planes, q1b, m1, m2;
+  int pz;  // 3-D tiles: this lane's plane of the tile (its window slab)
   unsigned cs_rcp, w1_rcp;  // udiv_rcp of cs and W1
   long long ncp, nsamp, plane_cp;
   bool active;
@@ -230,9 +231,8 @@ __device__ __attribute__((always_inline)) void tile_pass(const TileCtx& x, int c
 
   // ---- stage 2: rows of this wave, sliding window over k2 ----
   if (x.rb < x.re && x.active) {
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kLanes);
-    // 3-D tiles: wave w reads plane w; 2-D tiles: all waves read plane 0
-    const float* G = x.smem + (x.tab3 != nullptr ? static_cast<size_t>(wave) * x.nch * cs : 0);
+    // 3-D tiles: each lane reads its plane's slab; 2-D tiles: the one plane
+    const float* G = x.smem + (x.tab3 != nullptr ? static_cast<size_t>(x.pz) * x.nch * cs : 0);
     float V[P2 + 1][NCH];
     auto column = [&](int k2, float (&v)[NCH]) {
       const float* gr = G + (k2 - x.lo2) * x.W1 + x.q1b;
@@ -345,6 +345,9 @@ __device__ __attribute__((always_inline)) void batch_tile(const BGrid& g, const 
   x.nch = min(kBC, g.nout) + (nurbs ? 1 : 0);
   // 3-D grids: the tile holds kWaves consecutive samples j3 (from
   // kWaves * t.bz), one per wave, and every wave walks all rows of the chunk;
+  // a tile 32 or 16 samples wide packs 2 or 4 planes into the lane groups of
+  // a wave, which then walks a half or a quarter of the rows (the same rows
+  // in every group, so spans and B2 rows stay uniform over the wave);
   // 2-D grids: one plane, the waves split the chunk's rows
   const bool zpack = g.nin == 3;
   x.j3_0 = zpack ? t.bz * kWaves : 0;
@@ -364,16 +367,30 @@ __device__ __attribute__((always_inline)) void batch_tile(const BGrid& g, const 
   x.out = global_ptr(g.out);
   x.n1 = g.n1;
   x.p3 = g.p3;
-  x.rb = zpack ? (wave < x.planes ? r0 : r1) : r0 + wave * g.rows_per_wave;
-  x.re = zpack ? r1 : min(x.rb + g.rows_per_wave, r1);
-  x.j3 = x.j3_0 + (zpack ? wave : 0);
+  const int width = min(kLanes, g.m1 - j1_0);  // samples j1 of the tile
+  const int groups = zpack ? (width <= 16 ? 4 : width <= 32 ? 2 : 1) : 1;
+  const int group_lanes = kLanes / groups;
+  const int grp = lane / group_lanes, li = lane - grp * group_lanes;
+  const int slot0 = wave * groups;  // the wave's first plane slot
+  x.pz = zpack ? (slot0 + grp) % kWaves : 0;
+  if (zpack) {
+    const int part = slot0 / kWaves;  // the wave's share of the rows
+    const int len = (r1 - r0 + groups - 1) / groups;
+    const bool any = slot0 % kWaves < x.planes;
+    x.rb = any ? min(r0 + part * len, r1) : r1;
+    x.re = any ? min(r0 + (part + 1) * len, r1) : r1;
+  } else {
+    x.rb = r0 + wave * g.rows_per_wave;
+    x.re = min(x.rb + g.rows_per_wave, r1);
+  }
+  x.j3 = x.j3_0 + x.pz;
   x.m1 = g.m1;
   x.m2 = g.m2;
   x.ncp = g.ncp;
   x.nsamp = g.nsamp;
   x.plane_cp = static_cast<long long>(g.n1) * g.n2;
-  x.active = (j1_0 + lane) < g.m1;
-  x.j1 = x.active ? j1_0 + lane : g.m1 - 1;
+  x.active = zpack ? (li < width && x.pz < x.planes) : (j1_0 + lane) < g.m1;
+  x.j1 = x.active ? j1_0 + (zpack ? li : lane) : g.m1 - 1;
   x.q1b = span1[x.j1] - p1 - x.lo1;
   x.list = smem + list_at;
   x.nk = 0;
