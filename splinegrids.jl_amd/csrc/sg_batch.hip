@@ -47,6 +47,16 @@ namespace {
 
 
 
+// A wave walks at most kRowSegs * kLanes rows of a tile: the chunk of a 3-D
+// tile has at most kWaves * kMaxBatchRows rows.
+constexpr int kMaxBatchRows = 32;
+constexpr int kRowSegs = 2;
+static_assert(kWaves * kMaxBatchRows <= kRowSegs * kLanes, "row tables too small");
+
+__device__ __attribute__((always_inline)) inline float readlane_f(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
 // Everything a channel-block pass of a tile needs, set up once per tile.
 struct TileCtx {
   Global<const int32_t>* span3;
@@ -55,8 +65,6 @@ struct TileCtx {
   Global<const float>* W;
   Global<float>* out;
   float* smem;
-  Constant<const float>* tab2;    // [m2][P2+1] B2 rows, read with scalar loads
-  Constant<const int32_t>* span2;  // [m2] their spans
   const float* list;  // 3-D tiles: the plane list (build_plane_list)
   int nk;             // ... and its length
   int n1, p3, nch, cs, W1, lo1, lo2, rb, re, j3_0, j3, planes, q1b, m1, m2;
@@ -66,6 +74,12 @@ struct TileCtx {
   bool active;
   int j1;
   float b1[kMaxStreamDegree + 1];
+  // the wave's rows rb .. re - 1 (at most kRowSegs * kLanes of them): lane l
+  // holds the span and B2 row of row rb + l in segment 0 and of row
+  // rb + kLanes + l in segment 1; the row loop reads them with readlane, so
+  // a row costs no memory access before its window sums
+  int rspan[kRowSegs];
+  float rtab[kRowSegs][kMaxStreamDegree + 1];
 };
 
 // Window indices are below kMaxLds / 4 = 2^14, so udiv_fast(n, d) is exact
@@ -230,45 +244,47 @@ __device__ __attribute__((always_inline)) void tile_pass(const TileCtx& x, int c
   __syncthreads();
 
   // ---- stage 2: rows of this wave, sliding window over k2 ----
-  if (x.rb < x.re && x.active) {
+  // The loop is uniform over the wave (the row tables are read with
+  // readlane, which needs every lane's register); lanes outside the tile
+  // compute on clamped in-window indices and do not store.
+  if (x.rb < x.re) {
     // 3-D tiles: each lane reads its plane's slab; 2-D tiles: the one plane
     const float* G = x.smem + (x.tab3 != nullptr ? static_cast<size_t>(x.pz) * x.nch * cs : 0);
     float V[P2 + 1][NCH];
+    // all (P1 + 1) * NCH window reads of a column in flight before the sums
     auto column = [&](int k2, float (&v)[NCH]) {
       const float* gr = G + (k2 - x.lo2) * x.W1 + x.q1b;
+      float t[NCH][P1 + 1];
+#pragma unroll
+      for (int ch = 0; ch < NCH; ++ch)
+#pragma unroll
+        for (int q = 0; q <= P1; ++q) t[ch][q] = gr[ch * cs + q];
 #pragma unroll
       for (int ch = 0; ch < NCH; ++ch) {
         float s = 0.f;
 #pragma unroll
-        for (int q = 0; q <= P1; ++q) s += x.b1[q] * gr[ch * cs + q];
+        for (int q = 0; q <= P1; ++q) s += x.b1[q] * t[ch][q];
         v[ch] = s;
       }
     };
-    // Span and B2 row of each row from scalar loads issued one row ahead,
-    // rows in pairs with two register sets (no copies of prefetched values);
-    // the pointers stay on the last row at the end.
-    Constant<const int32_t>* sp = x.span2 + x.rb;
-    Constant<const float>* tp = x.tab2 + static_cast<size_t>(x.rb) * (P2 + 1);
-    auto fetch = [&](int& s2, float (&bb)[P2 + 1]) {
-      s2 = *sp;
-#pragma unroll
-      for (int q = 0; q <= P2; ++q) bb[q] = tp[q];
-    };
-    auto advance = [&](bool more) {
-      if (more) {
-        ++sp;
-        tp += P2 + 1;
-      }
-    };
-    int sA, sB;
-    float bA[P2 + 1], bB[P2 + 1];
-    fetch(sA, bA);  // row rb
-    int cur = sA;
+    int cur = __builtin_amdgcn_readlane(x.rspan[0], 0);
 #pragma unroll
     for (int q = 0; q <= P2; ++q) column(cur - P2 + q, V[q]);
     long long o = static_cast<long long>(x.j1) + static_cast<long long>(x.m1) * x.m2 * x.j3 +
                   x.nsamp * c0 + static_cast<long long>(x.m1) * x.rb;
-    auto row = [&](int s2, const float (&bb)[P2 + 1]) {
+#pragma unroll 1
+    for (int sgm = 0; sgm < kRowSegs; ++sgm) {
+      const int nr = min(kLanes, x.re - x.rb - sgm * kLanes);  // rows of the segment
+      if (nr <= 0) break;
+      const int rs = sgm == 0 ? x.rspan[0] : x.rspan[1];
+      float rt[P2 + 1];
+#pragma unroll
+      for (int q = 0; q <= P2; ++q) rt[q] = sgm == 0 ? x.rtab[0][q] : x.rtab[1][q];
+    for (int l = 0; l < nr; ++l) {
+      const int s2 = __builtin_amdgcn_readlane(rs, l);
+      float bb[P2 + 1];
+#pragma unroll
+      for (int q = 0; q <= P2; ++q) bb[q] = readlane_f(rt[q], l);
       if (s2 != cur) {
         if (s2 - cur > P2) {
           cur = s2;
@@ -294,20 +310,13 @@ __device__ __attribute__((always_inline)) void tile_pass(const TileCtx& x, int c
         num[ch] = s;
       }
       const float inv = NURBS ? nurbs_recip(num[NC]) : 1.f;
+      if (x.active) {
 #pragma unroll
-      for (int c = 0; c < NC; ++c) x.out[o + x.nsamp * c] = NURBS ? num[c] * inv : num[c];
+        for (int c = 0; c < NC; ++c) x.out[o + x.nsamp * c] = NURBS ? num[c] * inv : num[c];
+      }
       o += x.m1;
-    };
-    int j2 = x.rb;
-    for (; j2 + 1 < x.re; j2 += 2) {
-      advance(true);
-      fetch(sB, bB);  // row j2 + 1
-      row(sA, bA);
-      advance(j2 + 2 < x.re);
-      fetch(sA, bA);  // row j2 + 2 (row j2 + 1 again after the last pair)
-      row(sB, bB);
     }
-    if (j2 < x.re) row(sA, bA);
+    }
   }
   __syncthreads();  // G is rewritten by the next channel block / tile
 }
@@ -358,8 +367,6 @@ __device__ __attribute__((always_inline)) void batch_tile(const BGrid& g, const 
   const size_t list_at =
       (static_cast<size_t>(zpack ? kWaves : 1) * x.nch * x.cs + 3) & ~size_t(3);
   x.smem = smem;
-  x.tab2 = constant_ptr(g.tab2);
-  x.span2 = constant_ptr(g.span2);
   x.span3 = zpack ? global_ptr(g.span3) : nullptr;
   x.tab3 = zpack ? global_ptr(g.tab3) : nullptr;
   x.P = global_ptr(g.P);
@@ -383,7 +390,22 @@ __device__ __attribute__((always_inline)) void batch_tile(const BGrid& g, const 
     x.rb = r0 + wave * g.rows_per_wave;
     x.re = min(x.rb + g.rows_per_wave, r1);
   }
+  x.rb = wave_uniform(x.rb);  // scalar row bounds for the row loops
+  x.re = wave_uniform(x.re);
   x.j3 = x.j3_0 + x.pz;
+  // the row tables, loaded before stage 1 so that their latency hides under
+  // it; rows past re - 1 repeat row re - 1 (re >= 1, so every read is a row
+  // of the grid, also for a wave with no rows)
+  {
+    const auto tab2 = global_ptr(g.tab2);
+#pragma unroll
+    for (int sgm = 0; sgm < kRowSegs; ++sgm) {
+      const int r = min(x.rb + sgm * kLanes + lane, x.re - 1);
+      x.rspan[sgm] = span2[r];
+#pragma unroll
+      for (int q = 0; q <= P2; ++q) x.rtab[sgm][q] = tab2[static_cast<size_t>(r) * (P2 + 1) + q];
+    }
+  }
   x.m1 = g.m1;
   x.m2 = g.m2;
   x.ncp = g.ncp;
@@ -422,9 +444,10 @@ __device__ __attribute__((always_inline)) void batch_tile_p2(const BGrid& g, con
   }
 }
 
-// At most 80 VGPRs (six waves per SIMD, without spills): with the LDS share
-// of six workgroups per CU, all six stay resident.
-__global__ __launch_bounds__(kThreads, 6) void batch_persistent(const BGrid* __restrict__ grids,
+// At most 128 VGPRs (four waves per SIMD): the row loop keeps a window
+// column's LDS reads in flight together and the row tables in registers,
+// which outweighs the residency of six workgroups per CU.
+__global__ __launch_bounds__(kThreads, 4) void batch_persistent(const BGrid* __restrict__ grids,
                                                              const BTile* __restrict__ tiles,
                                                              int* counters) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -598,6 +621,9 @@ int build_mma_tiles(sg_batch* b) {
 // that order, each grid's tiles row by row, so neighbouring tiles share
 // control-point planes in L2.
 int build_tiles(sg_batch* b, int rows_per_wave, int blocks_per_cu) {
+  // the kernel's row tables hold the rows of a wave for at most this many
+  if (rows_per_wave < 1 || rows_per_wave > kMaxBatchRows)
+    return fail(SG_ERR_INVALID, "batch rows per wave out of range");
   const size_t cap = lds_cap(blocks_per_cu);
   std::vector<std::vector<BTile>> grid_tiles;
   std::vector<long long> grid_cost;
