@@ -452,19 +452,24 @@ __global__ __launch_bounds__(kThreads, 4) void batch_persistent(const BGrid* __r
                                                              int* counters) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   float* smem = reinterpret_cast<float*>(smem_raw);
-  __shared__ int s_tile;
+  __shared__ int s_tile[2];
   const int lane = threadIdx.x & (kLanes - 1);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kLanes);
   const int home = static_cast<int>(blockIdx.x) % kXcds;
   int visited = 0;  // thread 0: queues found empty so far
+  int slot = 0;     // s_tile[slot]: the tile to take next
+  if (threadIdx.x == 0) s_tile[0] = pop_tile(counters, home, visited);
   for (;;) {
-    if (threadIdx.x == 0) s_tile = pop_tile(counters, home, visited);
     __syncthreads();
     // every thread read the same value: the exit is uniform, and with the
     // index in a scalar register the tile and grid records are scalar loads
     // (branches on them stay scalar branches)
-    const int ti = wave_uniform(s_tile);
+    const int ti = wave_uniform(s_tile[slot]);
     if (ti < 0) break;
+    // thread 0 claims the following tile now and publishes it once this one
+    // is done, so the queue's atomic round trip overlaps this tile's work
+    int next = -1;
+    if (threadIdx.x == 0) next = pop_tile(counters, home, visited);
     const BTile t = tiles[ti];
     const BGrid g = grids[wave_uniform(t.grid)];
     switch (g.p1) {
@@ -475,7 +480,10 @@ __global__ __launch_bounds__(kThreads, 4) void batch_persistent(const BGrid* __r
       case 4: batch_tile_p2<4>(g, t, smem, lane, wave); break;
       default: batch_tile_p2<5>(g, t, smem, lane, wave); break;
     }
-    // batch_tile ends with a barrier, so s_tile may be overwritten now
+    // every thread read s_tile[slot] before the barriers of the tile; the
+    // barrier at the top of the loop publishes the other slot
+    if (threadIdx.x == 0) s_tile[slot ^ 1] = next;
+    slot ^= 1;
   }
 }
 
