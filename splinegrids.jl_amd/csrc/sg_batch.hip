@@ -30,15 +30,14 @@ struct BGrid {
   float* out;
   const float* tab1;
   const float* tab2;
-  const float* tab3;
   const int32_t* span1;
   const int32_t* span2;
-  const int32_t* span3;
+  const float* lists;  // 3-D grids: plane list of each tile row bz, list_stride floats apart
   int nin, nout, nurbs;
   int n1, n2, p1, p2, p3;
   int m1, m2, m3;
   int rows_per_wave;
-  int pad;
+  int list_stride;
   long long ncp, nsamp;
 };
 
@@ -59,13 +58,13 @@ __device__ __attribute__((always_inline)) inline float readlane_f(float v, int l
 
 // Everything a channel-block pass of a tile needs, set up once per tile.
 struct TileCtx {
-  Global<const int32_t>* span3;
-  Global<const float>* tab3;  // null for 2-D grids
   Global<const float>* P;
   Global<const float>* W;
   Global<float>* out;
   float* smem;
-  const float* list;  // 3-D tiles: the plane list (build_plane_list)
+  // 3-D tiles: the plane list (plane_list_host), read with scalar loads;
+  // null for 2-D tiles
+  Constant<const float>* list;
   int nk;             // ... and its length
   int n1, p3, nch, cs, W1, lo1, lo2, rb, re, j3_0, j3, planes, q1b, m1, m2;
   int pz;  // 3-D tiles: this lane's plane of the tile (its window slab)
@@ -85,43 +84,6 @@ struct TileCtx {
 // Window indices are below kMaxLds / 4 = 2^14, so udiv_fast(n, d) is exact
 // for them (n * d < 2^28).
 
-// Plane list of a 3-D tile: the union of the dimension-3 bands of its
-// planes, one entry per control-point plane k3 that any of them needs:
-// {w[0..3], k3}, where w[pz] is B3 of plane pz at k3 (0 outside its band).
-// Every k3 is loaded once and serves all planes.  An extra entry (zero
-// weights, a valid k3) lets stage 1 take the entries in pairs.
-// (kEntry floats per entry: weights (16-byte aligned), k3 at kEntryK, padding)
-
-// Builds the list into `list` (16-byte aligned LDS) and returns its length.
-// The spans of the planes are non-decreasing, so band q adds the planes
-// [max(s3[q] - p3, s3[q-1] + 1), s3[q]] to those of the bands before it.
-__device__ inline int build_plane_list(const TileCtx& x, float* list) {
-  const int p3 = x.p3;
-  int s3[kWaves], lo[kWaves], base[kWaves + 1];
-#pragma unroll
-  for (int q = 0; q < kWaves; ++q) s3[q] = wave_uniform(x.span3[x.j3_0 + min(q, x.planes - 1)]);
-  base[0] = 0;
-#pragma unroll
-  for (int q = 0; q < kWaves; ++q) {
-    lo[q] = q == 0 ? s3[0] - p3 : max(s3[q] - p3, s3[q - 1] + 1);
-    base[q + 1] = base[q] + max(0, s3[q] - lo[q] + 1);
-  }
-  const int nk = base[kWaves];
-  for (int t = threadIdx.x; t < (nk + 1) * kWaves; t += kThreads) {
-    const int i = t / kWaves, pz = t % kWaves;
-    int k3 = s3[kWaves - 1];  // the padding entry's plane
-#pragma unroll
-    for (int q = 0; q < kWaves; ++q)
-      if (i >= base[q] && i < base[q + 1]) k3 = lo[q] + i - base[q];
-    const int d = k3 - (s3[pz] - p3);
-    const bool in = i < nk && pz < x.planes && d >= 0 && d <= p3;
-    const float w = x.tab3[static_cast<size_t>(x.j3_0 + min(pz, x.planes - 1)) * (p3 + 1) +
-                           min(max(d, 0), p3)];
-    list[i * kEntry + pz] = in ? w : 0.f;
-    if (pz == 0) reinterpret_cast<int*>(list)[i * kEntry + kEntryK] = k3;
-  }
-  return nk;
-}
 
 // Stage 1 of a 2-D tile: the window of channels [c0, c0 + NC) (times the
 // weight, and the weight last, for NURBS) copied into LDS, U elements per
@@ -167,7 +129,7 @@ __device__ __attribute__((always_inline)) void stage1_3d(const TileCtx& x, int c
   constexpr int NCH = NC + (NURBS ? 1 : 0);
   constexpr int U = NCH <= 2 ? 2 : 1;
   const int cs = x.cs, nk = x.nk;
-  const float* list = x.list;
+  Constant<const float>* list = x.list;
   for (int e0 = threadIdx.x; e0 < cs; e0 += U * kThreads) {
     long long cpi[U];
     int e[U];
@@ -189,9 +151,9 @@ __device__ __attribute__((always_inline)) void stage1_3d(const TileCtx& x, int c
       f4 wv[2];
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
-        const float* ent = list + (i + h) * kEntry;
-        wv[h] = *reinterpret_cast<const f4*>(ent);
-        const long long plane = x.plane_cp * reinterpret_cast<const int*>(ent)[kEntryK];
+        Constant<const float>* ent = list + (i + h) * kEntry;
+        wv[h] = *reinterpret_cast<Constant<const f4>*>(ent);
+        const long long plane = x.plane_cp * reinterpret_cast<Constant<const int>*>(ent)[kEntryK];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
           const long long off = cpi[u] + plane;
@@ -237,7 +199,7 @@ __device__ __attribute__((always_inline)) void tile_pass(const TileCtx& x, int c
   constexpr int NCH = NC + (NURBS ? 1 : 0);
   const int cs = x.cs;
   // ---- stage 1: band windows, contracted along dimension 3 ----
-  if (x.tab3 == nullptr)
+  if (x.list == nullptr)
     stage1_2d<NC, NURBS>(x, c0);
   else
     stage1_3d<NC, NURBS>(x, c0);
@@ -249,7 +211,7 @@ __device__ __attribute__((always_inline)) void tile_pass(const TileCtx& x, int c
   // compute on clamped in-window indices and do not store.
   if (x.rb < x.re) {
     // 3-D tiles: each lane reads its plane's slab; 2-D tiles: the one plane
-    const float* G = x.smem + (x.tab3 != nullptr ? static_cast<size_t>(x.pz) * x.nch * cs : 0);
+    const float* G = x.smem + (x.list != nullptr ? static_cast<size_t>(x.pz) * x.nch * cs : 0);
     float V[P2 + 1][NCH];
     // all (P1 + 1) * NCH window reads of a column in flight before the sums
     auto column = [&](int k2, float (&v)[NCH]) {
@@ -361,14 +323,9 @@ __device__ __attribute__((always_inline)) void batch_tile(const BGrid& g, const 
   const bool zpack = g.nin == 3;
   x.j3_0 = zpack ? t.bz * kWaves : 0;
   x.planes = zpack ? min(kWaves, g.m3 - x.j3_0) : 1;
-  // [planes][nch][cs] window slabs (weight last; space for kWaves planes is
-  // reserved for 3-D grids, as the host's LDS size assumes), then the plane
-  // list of 3-D grids, 16-byte aligned
-  const size_t list_at =
-      (static_cast<size_t>(zpack ? kWaves : 1) * x.nch * x.cs + 3) & ~size_t(3);
+  // LDS: [planes][nch][cs] window slabs (weight last; space for kWaves planes
+  // is reserved for 3-D grids, as the host's LDS size assumes)
   x.smem = smem;
-  x.span3 = zpack ? global_ptr(g.span3) : nullptr;
-  x.tab3 = zpack ? global_ptr(g.tab3) : nullptr;
   x.P = global_ptr(g.P);
   x.W = global_ptr(g.W);
   x.out = global_ptr(g.out);
@@ -414,12 +371,8 @@ __device__ __attribute__((always_inline)) void batch_tile(const BGrid& g, const 
   x.active = zpack ? (li < width && x.pz < x.planes) : (j1_0 + lane) < g.m1;
   x.j1 = x.active ? j1_0 + (zpack ? li : lane) : g.m1 - 1;
   x.q1b = span1[x.j1] - p1 - x.lo1;
-  x.list = smem + list_at;
-  x.nk = 0;
-  if (zpack) {  // uniform over the workgroup
-    x.nk = build_plane_list(x, smem + list_at);
-    __syncthreads();
-  }
+  x.list = zpack ? constant_ptr(g.lists) + static_cast<size_t>(t.bz) * g.list_stride : nullptr;
+  x.nk = zpack ? reinterpret_cast<Constant<const int>*>(x.list)[kEntryN] : 0;
 #pragma unroll
   for (int q = 0; q <= P1; ++q) x.b1[q] = tab1[static_cast<size_t>(x.j1) * (P1 + 1) + q];
   for (int c0 = 0; c0 < g.nout; c0 += kBC) {
@@ -502,6 +455,7 @@ struct sg_batch {
   std::vector<BGrid> hgrids;
   BGrid* dgrids = nullptr;
   BTile* dtiles = nullptr;
+  float* dlists = nullptr;  // plane lists of the 3-D grids' tile rows
   int* counters = nullptr;  // kQueueInts: queue counters, then queue starts
   // MFMA tiers: one argument record per (grid, channel block) and their tiles
   // in queues of their own; mma_ok when every grid takes the tiers
@@ -536,6 +490,7 @@ void free_batch(sg_batch* b) {
   for (Plan* p : b->plans) destroy_plan(p);
   if (b->dgrids) (void)hipFree(b->dgrids);
   if (b->dtiles) (void)hipFree(b->dtiles);
+  if (b->dlists) (void)hipFree(b->dlists);
   if (b->counters) (void)hipFree(b->counters);
   if (b->dmargs) (void)hipFree(b->dmargs);
   if (b->dtiles_m) (void)hipFree(b->dtiles_m);
@@ -544,6 +499,39 @@ void free_batch(sg_batch* b) {
 }
 
 
+
+// The plane lists of every 3-D grid's tile rows (kWaves samples j3 each),
+// built on the host from the FP32 order-0 table the kernels use (copied back
+// once), in one device buffer.
+hipError_t upload_plane_lists(sg_batch* b, hipStream_t s) {
+  std::vector<float> lists;
+  std::vector<size_t> at(b->plans.size(), 0);
+  std::vector<float> tab;
+  hipError_t e = hipStreamSynchronize(s);  // the plans' tables are built on s
+  for (size_t i = 0; i < b->plans.size() && e == hipSuccess; ++i) {
+    const Plan& pl = *b->plans[i];
+    if (pl.nin != 3) continue;
+    const DimDev& d3 = pl.dims[2];
+    tab.resize(static_cast<size_t>(d3.m) * (d3.p + 1));
+    e = hipMemcpy(tab.data(), d3.ktab32, tab.size() * sizeof(float), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) break;
+    const int stride = plane_list_floats(d3.p), nz = (d3.m + kWaves - 1) / kWaves;
+    at[i] = lists.size();
+    lists.resize(lists.size() + static_cast<size_t>(nz) * stride);
+    for (int z = 0; z < nz; ++z)
+      plane_list_host(d3.h_span.data(), tab.data(), d3.p, z * kWaves,
+                      std::min(kWaves, d3.m - z * kWaves), lists.data() + at[i] + size_t(z) * stride);
+    b->hgrids[i].list_stride = stride;
+  }
+  if (e != hipSuccess || lists.empty()) return e;
+  e = hipMalloc(&b->dlists, lists.size() * sizeof(float));
+  if (e == hipSuccess)
+    e = hipMemcpy(b->dlists, lists.data(), lists.size() * sizeof(float), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return e;
+  for (size_t i = 0; i < b->plans.size(); ++i)
+    if (b->plans[i]->nin == 3) b->hgrids[i].lists = b->dlists + at[i];
+  return hipSuccess;
+}
 
 // MFMA tiers: argument records and tiles of every grid (the grids' arrays
 // are filled into the records by sg_batch_set_arrays).  Leaves mma_ok false
@@ -773,8 +761,6 @@ int sg_batch_create(const sg_grid_desc* grids, int32_t n_grids, void* stream, sg
       const DimDev& d3 = pl.dims[2];
       g.p3 = d3.p;
       g.m3 = d3.m;
-      g.tab3 = d3.ktab32;
-      g.span3 = d3.span;
       g.ncp *= d3.n;
       g.nsamp *= d3.m;
     }
@@ -784,6 +770,7 @@ int sg_batch_create(const sg_grid_desc* grids, int32_t n_grids, void* stream, sg
     if (e == hipSuccess) e = hipMalloc(&b->counters, kQueueInts * sizeof(int));
     if (e != hipSuccess) return bail(hip_fail(e, "batch allocation"));
   }
+  if ((e = upload_plane_lists(b, s)) != hipSuccess) return bail(hip_fail(e, "batch plane lists"));
   int rc = build_tiles(b, 16, 3);
   if (rc != SG_OK) return bail(rc);
   if (n_grids > 0 && (rc = build_mma_tiles(b)) != SG_OK) return bail(rc);

@@ -304,10 +304,31 @@ GridTiling grid_tiling(const Plan& pl, int rows) {
   }
   const size_t nch = std::min(pl.nout, kBC) + (pl.nurbs ? 1 : 0);
   const size_t planes = pl.nin == 3 ? kWaves : 1;
-  // window slabs, alignment and the plane list of 3-D tiles
-  const size_t list = pl.nin == 3 ? 3 + plane_list_floats(pl.dims[2].p) : 0;
-  t.lds = (planes * nch * t.w1 * t.w2 + list) * sizeof(float);
+  t.lds = planes * nch * t.w1 * t.w2 * sizeof(float);  // the window slabs
   return t;
+}
+
+int plane_list_host(const int32_t* span3, const float* tab3, int p3, int j3_0, int planes,
+                    float* list) {
+  const int cap = plane_list_floats(p3);
+  std::fill(list, list + cap, 0.f);
+  std::vector<int> ks;
+  for (int pz = 0; pz < planes; ++pz)
+    for (int d = 0; d <= p3; ++d) ks.push_back(span3[j3_0 + pz] - p3 + d);
+  std::sort(ks.begin(), ks.end());
+  ks.erase(std::unique(ks.begin(), ks.end()), ks.end());
+  const int nk = static_cast<int>(ks.size());
+  auto put_k = [&](int i, int k3) { std::memcpy(list + i * kEntry + kEntryK, &k3, sizeof(int)); };
+  for (int i = 0; i < nk; ++i) {
+    put_k(i, ks[i]);
+    for (int pz = 0; pz < planes; ++pz) {
+      const int d = ks[i] - (span3[j3_0 + pz] - p3);
+      if (d >= 0 && d <= p3) list[i * kEntry + pz] = tab3[static_cast<size_t>(j3_0 + pz) * (p3 + 1) + d];
+    }
+  }
+  put_k(nk, ks.back());  // padding: zero weights, a plane of the grid
+  std::memcpy(list + kEntryN, &nk, sizeof(int));
+  return nk;
 }
 
 void pack_queues(const std::vector<std::vector<BTile>>& grid_tiles,

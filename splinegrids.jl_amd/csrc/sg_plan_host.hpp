@@ -62,12 +62,25 @@ void band_lists(const DimDev& D, std::vector<int32_t>& ptr, std::vector<int32_t>
 // ---- the batch's persistent launch (sg_batch.hip) ----
 
 constexpr int kBC = 4;  // value channels per pass; the weight is slab 4
-// 3-D tiles' plane list: entries of kEntry floats, weights then k3 at kEntryK
+// 3-D tiles' plane list: entries of kEntry floats, weights then k3 at
+// kEntryK; entry 0 holds the list's length at kEntryN
 constexpr int kEntry = 8;
 constexpr int kEntryK = 4;
+constexpr int kEntryN = 5;
 __host__ __device__ constexpr int plane_list_floats(int p3) {
   return (kWaves * (p3 + 1) + 1) * kEntry;
 }
+
+// The plane list of the 3-D batch tile whose kWaves samples j3 start at j3_0
+// (`planes` <= kWaves of them inside the grid): one entry per control-point
+// plane k3 in the union of their dimension-3 bands, in increasing k3,
+// {w[0..kWaves), k3}, where w[pz] is B3 of sample j3_0 + pz at k3 (0 outside
+// its band), then one padding entry with zero weights and the last k3, so
+// that entries can be taken in pairs.  span3 / tab3: the dimension's spans
+// and FP32 order-0 table [m3][p3 + 1].  Writes plane_list_floats(p3) floats
+// (the rest zero) and returns the length.
+int plane_list_host(const int32_t* span3, const float* tab3, int p3, int j3_0, int planes,
+                    float* list);
 
 // LDS one workgroup may take when blocks_per_cu of them share a CU.
 size_t lds_cap(int blocks_per_cu);

@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <cstdlib>
 #include <random>
 #include <string>
@@ -188,8 +189,52 @@ void check_bands(const DimDev& D, const std::string& what) {
     }
 }
 
+// the batch's plane lists of a 3-D grid, every tile row: each band entry
+// of each sample exactly once with its weight, k3 increasing, the padding
+// entry, the stored length, nothing written past the list
+void check_plane_lists(const Plan& pl, const std::string& nm) {
+  const DimDev& D = pl.dims[2];
+  const int p = D.p, stride = plane_list_floats(p);
+  std::vector<float> tab(static_cast<size_t>(D.m) * (p + 1));
+  for (size_t i = 0; i < tab.size(); ++i) tab[i] = 0.5f + static_cast<float>(i);  // distinct, nonzero
+  std::vector<float> list(stride + kEntry, -7.f);
+  for (int j0 = 0; j0 < D.m; j0 += kWaves) {
+    const int planes = std::min(kWaves, D.m - j0);
+    const int nk = plane_list_host(D.h_span.data(), tab.data(), p, j0, planes, list.data());
+    const std::string w = nm + " plane list j3 " + std::to_string(j0);
+    require(nk >= p + 1 && nk <= kWaves * (p + 1), w + ": length");
+    int n_stored;
+    std::memcpy(&n_stored, list.data() + kEntryN, sizeof(int));
+    require(n_stored == nk, w + ": stored length");
+    for (int i = 0; i < kEntry; ++i) require(list[stride + i] == -7.f, w + ": nothing past the list");
+    auto k_of = [&](int i) {
+      int k;
+      std::memcpy(&k, list.data() + i * kEntry + kEntryK, sizeof(int));
+      return k;
+    };
+    for (int i = 0; i + 1 < nk; ++i) require(k_of(i) < k_of(i + 1), w + ": k3 increasing");
+    require(k_of(nk) == k_of(nk - 1), w + ": padding plane");
+    for (int pz = 0; pz < kWaves; ++pz) require(list[nk * kEntry + pz] == 0.f, w + ": padding weights");
+    for (int pz = 0; pz < kWaves; ++pz) {
+      int found = 0;
+      for (int i = 0; i < nk; ++i) {
+        const float v = list[i * kEntry + pz];
+        const int d = k_of(i) - (pz < planes ? D.h_span[j0 + pz] - p : 0);
+        if (pz < planes && d >= 0 && d <= p) {
+          require(v == tab[static_cast<size_t>(j0 + pz) * (p + 1) + d], w + ": weight");
+          ++found;
+        } else {
+          require(v == 0.f, w + ": zero outside the band");
+        }
+      }
+      require(found == (pz < planes ? p + 1 : 0), w + ": every band entry once");
+    }
+  }
+}
+
 void check_plan(Plan& pl) {
   const std::string nm = name_of(pl);
+  if (pl.nin == 3 && pl.dims[2].m > 0) check_plane_lists(pl, nm);
   const bool adjm_ok = (pl.nin == 2 || pl.nin == 3) &&
                        (pl.nin == 2 || pl.dims[2].p <= kMaxStreamDegree);
   plan_tilings(pl, true, adjm_ok, [](const Plan&) { return 2048; });
