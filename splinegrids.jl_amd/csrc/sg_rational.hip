@@ -15,11 +15,19 @@
 //   R^(g) = (A^(g) - sum_{0 < b <= g} C(g, b) w^(b) R^(g-b)) / w,
 // C(g, b) = prod_d binom(g_d, b_d), over the orders g of the closure in
 // increasing total order, and writes the requested fields.
+//
+// Value and first derivatives (every requested order of total order <= 1)
+// take one fused pass instead (rat_grad_stream): the forward's tiles and
+// stage 1 over the homogeneous window (w P_c, w), a second window contracted
+// with B3' when d/dx3 is asked for, and in each row the sums A, w and their
+// first derivatives in registers, then R = A / w and
+// dR/dx_d = (A_d - w_d R) / w before the stores; nothing but the requested
+// fields goes through HBM.
 #include <algorithm>
 #include <array>
 #include <vector>
 
-#include "sg_internal.hpp"
+#include "sg_tiles.hpp"
 
 namespace sg {
 
@@ -39,6 +47,284 @@ struct RatArgs {
   unsigned char tg[kMaxTerms], tb[kMaxTerms], tr[kMaxTerms];
   float coef[kMaxTerms];
 };
+
+// ---- fused value and first derivatives ----
+
+template <typename T>
+struct RatGradArgs {
+  BandSrc<T> src;    // control points and weights, dimension 3 at order 0
+  const T* tab3d;    // dimension 3 at order 1 (3-D grids asking for d/dx3)
+  const T* tab1[2];  // dimension 1 at orders 0 and 1 [m1][p1+1]
+  const int32_t* span1;
+  int m1, m2, p1, c0, rows_per_wave, wx, ntx, nty, xcd_map;
+  long long nsamp;
+  int store[4];  // value, d/dx1, d/dx2, d/dx3: written or not
+};
+
+// One tile (the forward's tiles: kLanes * wx samples j1, a chunk of rows j2,
+// one sample j3) for channels [c0, c0 + NC): stage 1 stages the homogeneous
+// window G0 (B3 contraction) and, with D3, G1 (B3' contraction); each lane
+// walks its rows with the register windows V0 = b1 . G0, V0d = b1' . G0 and
+// V1 = b1 . G1 over k2, and a row's sums are A = B2 . V0, A_1 = B2 . V0d,
+// A_2 = B2' . V0, A_3 = B2 . V1 (the weight as channel NC of each).
+template <typename T, int NIN, int P2, int NC, bool D3>
+__global__ __launch_bounds__(kThreads) void rat_grad_stream(
+    RatGradArgs<T> a, const int32_t* __restrict__ span2, const T* __restrict__ tb2,
+    const T* __restrict__ tb2d, T* __restrict__ o_val, T* __restrict__ o_d1,
+    T* __restrict__ o_d2, T* __restrict__ o_d3) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  T* G = reinterpret_cast<T*>(smem);
+  constexpr int NCH = NC + 1;
+  constexpr int PM = kMaxStreamDegree;
+  const int lane = threadIdx.x & (kLanes - 1);
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kLanes);
+  const int wy = kWaves / a.wx;
+  const FwdTile tile = fwd_tile(a.xcd_map, a.ntx, a.nty, kLanes * a.wx, a.rows_per_wave * wy,
+                                a.m1, a.m2, NIN);
+  const int j1_0 = tile.j1_0, r0 = tile.r0, r1 = tile.r1, j3 = tile.j3;
+  const int lo1 = a.span1[j1_0] - a.p1;
+  const int W1 = a.span1[tile.j1_last] - lo1 + 1;
+  const int lo2 = span2[r0] - P2;
+  const int W2 = span2[r1 - 1] - lo2 + 1;
+  const int cs = W1 * W2;
+
+  stage1_band_window<T, NIN, NC, true>(G, a.src, j3, lo1, W1, lo2, W2);
+  if constexpr (D3) {
+    BandSrc<T> src = a.src;
+    src.tab3 = a.tab3d;
+    stage1_band_window<T, NIN, NC, true>(G + NCH * cs, src, j3, lo1, W1, lo2, W2);
+  }
+  __syncthreads();
+
+  const int wave_x = wave % a.wx, wave_y = wave / a.wx;
+  const int rb = r0 + wave_y * a.rows_per_wave;
+  const int re = min(rb + a.rows_per_wave, r1);
+  const int j1 = j1_0 + kLanes * wave_x + lane;
+  // no barrier follows: lanes past the grid, and waves without rows, leave
+  if (rb >= re || j1 >= a.m1) return;
+  const int p1 = a.p1;
+  const int q1b = a.span1[j1] - p1 - lo1;
+  T b1[2][PM + 1];  // zero past p1 (the clamped index re-reads the band's end)
+#pragma unroll
+  for (int o = 0; o < 2; ++o)
+#pragma unroll
+    for (int t = 0; t <= PM; ++t) {
+      const T v = a.tab1[o][static_cast<size_t>(j1) * (p1 + 1) + min(t, p1)];
+      b1[o][t] = t <= p1 ? v : T(0);
+    }
+
+  constexpr int NV = D3 ? 3 : 2;  // windows: V0, V0d (, V1)
+  T V[P2 + 1][NV][NCH];
+  auto column = [&](int k2, T (&v)[NV][NCH]) {
+    const T* g = G + (k2 - lo2) * W1 + q1b;
+    T x[D3 ? 2 : 1][NCH][PM + 1];  // every read of the column before the sums
+#pragma unroll
+    for (int h = 0; h < (D3 ? 2 : 1); ++h)
+#pragma unroll
+      for (int ch = 0; ch < NCH; ++ch)
+#pragma unroll
+        for (int t = 0; t <= PM; ++t) x[h][ch][t] = g[(h * NCH + ch) * cs + min(t, p1)];
+#pragma unroll
+    for (int ch = 0; ch < NCH; ++ch) {
+      T s0 = T(0), s1 = T(0), s2 = T(0);
+#pragma unroll
+      for (int t = 0; t <= PM; ++t) {
+        s0 += b1[0][t] * x[0][ch][t];
+        s1 += b1[1][t] * x[0][ch][t];
+        if constexpr (D3) s2 += b1[0][t] * x[D3 ? 1 : 0][ch][t];
+      }
+      v[0][ch] = s0;
+      v[1][ch] = s1;
+      if constexpr (D3) v[NV - 1][ch] = s2;
+    }
+  };
+  int cur = span2[rb];
+#pragma unroll
+  for (int q = 0; q <= P2; ++q) column(cur - P2 + q, V[q]);
+
+  long long o = static_cast<long long>(j1) + static_cast<long long>(a.m1) * a.m2 * j3 +
+                a.nsamp * a.c0 + static_cast<long long>(a.m1) * rb;
+  for (int j2 = rb; j2 < re; ++j2, o += a.m1) {
+    const int s2 = span2[j2];
+    if (s2 != cur) {
+      if (s2 - cur > P2) {
+        cur = s2;
+#pragma unroll
+        for (int q = 0; q <= P2; ++q) column(cur - P2 + q, V[q]);
+      } else {
+        while (cur < s2) {
+#pragma unroll
+          for (int q = 0; q < P2; ++q)
+#pragma unroll
+            for (int h = 0; h < NV; ++h)
+#pragma unroll
+              for (int ch = 0; ch < NCH; ++ch) V[q][h][ch] = V[q + 1][h][ch];
+          ++cur;
+          column(cur, V[P2]);
+        }
+      }
+    }
+    const T* bb0 = tb2 + static_cast<size_t>(j2) * (P2 + 1);
+    const T* bb1 = tb2d + static_cast<size_t>(j2) * (P2 + 1);
+    T A[NCH], A1[NCH], A2[NCH], A3[NCH];
+#pragma unroll
+    for (int ch = 0; ch < NCH; ++ch) {
+      T s0 = T(0), s1 = T(0), s2 = T(0), s3 = T(0);
+#pragma unroll
+      for (int q = 0; q <= P2; ++q) {
+        s0 += bb0[q] * V[q][0][ch];
+        s1 += bb0[q] * V[q][1][ch];
+        s2 += bb1[q] * V[q][0][ch];
+        if constexpr (D3) s3 += bb0[q] * V[q][NV - 1][ch];
+      }
+      A[ch] = s0;
+      A1[ch] = s1;
+      A2[ch] = s2;
+      A3[ch] = s3;
+    }
+    const T inv = T(1) / A[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      const T R = A[c] * inv;
+      const long long i = o + a.nsamp * c;
+      if (a.store[0]) o_val[i] = R;
+      if (a.store[1]) o_d1[i] = (A1[c] - A1[NC] * R) * inv;
+      if (a.store[2]) o_d2[i] = (A2[c] - A2[NC] * R) * inv;
+      if constexpr (D3)
+        if (a.store[3]) o_d3[i] = (A3[c] - A3[NC] * R) * inv;
+    }
+  }
+}
+
+template <typename T>
+using RatGradKernel = void (*)(RatGradArgs<T>, const int32_t*, const T*, const T*, T*, T*, T*,
+                               T*);
+
+template <typename T, int NIN, int P2, bool D3>
+RatGradKernel<T> pick_rat_grad_nc(int nc) {
+  switch (nc) {
+    case 1: return rat_grad_stream<T, NIN, P2, 1, D3>;
+    case 2: return rat_grad_stream<T, NIN, P2, 2, D3>;
+    case 3: return rat_grad_stream<T, NIN, P2, 3, D3>;
+  }
+  return nullptr;
+}
+
+template <typename T, int NIN, bool D3>
+RatGradKernel<T> pick_rat_grad_p2(int p2, int nc) {
+  switch (p2) {
+    case 0: return pick_rat_grad_nc<T, NIN, 0, D3>(nc);
+    case 1: return pick_rat_grad_nc<T, NIN, 1, D3>(nc);
+    case 2: return pick_rat_grad_nc<T, NIN, 2, D3>(nc);
+    case 3: return pick_rat_grad_nc<T, NIN, 3, D3>(nc);
+    case 4: return pick_rat_grad_nc<T, NIN, 4, D3>(nc);
+    case 5: return pick_rat_grad_nc<T, NIN, 5, D3>(nc);
+  }
+  return nullptr;
+}
+
+template <typename T>
+const T* order_table(const DimDev& d, int order);
+template <>
+const float* order_table<float>(const DimDev& d, int order) {
+  return d.ktab32 + static_cast<size_t>(order) * d.m * (d.p + 1);
+}
+template <>
+const double* order_table<double>(const DimDev& d, int order) {
+  return d.ktab64 + static_cast<size_t>(order) * d.m * (d.p + 1);
+}
+
+constexpr int kRatGradMaxNC = 3;  // channels per launch (the weight comes on top)
+
+// The fused pass applies when every field has total order <= 1, no field
+// repeats, the grid takes the streaming kernels and the windows fit; returns
+// the channels per launch, or 0.
+int rat_grad_channels(const Plan& pl, int nf, const int32_t* orders, size_t elem) {
+  if (!pl.stream_ok || (pl.nin != 2 && pl.nin != 3)) return 0;
+  int seen[4] = {0, 0, 0, 0};
+  bool d3 = false;
+  for (int f = 0; f < nf; ++f) {
+    int total = 0, kind = 0;
+    for (int d = 0; d < pl.nin; ++d) {
+      const int o = orders[f * pl.nin + d];
+      total += o;
+      if (o == 1) kind = d + 1;
+    }
+    if (total > 1 || seen[kind]++) return 0;
+    d3 = d3 || kind == 3;
+  }
+  const Tiling& t = pl.fwd;
+  for (int nc = std::min(kRatGradMaxNC, pl.nout); nc >= 1; --nc) {
+    const size_t lds = static_cast<size_t>(d3 ? 2 : 1) * (nc + 1) * t.w1max * t.w2max * elem;
+    if (lds <= static_cast<size_t>(kMaxLds)) return nc;
+  }
+  return 0;
+}
+
+template <typename T>
+hipError_t rational_first_order(const Plan& pl, int nf, const int32_t* orders, const T* P,
+                                const T* W, T* const* outs, int nc_max, hipStream_t s) {
+  const DimDev& d1 = pl.dims[0];
+  const DimDev& d2 = pl.dims[1];
+  RatGradArgs<T> a{};
+  T* out[4] = {nullptr, nullptr, nullptr, nullptr};
+  for (int f = 0; f < nf; ++f) {
+    int kind = 0;
+    for (int d = 0; d < pl.nin; ++d)
+      if (orders[f * pl.nin + d] == 1) kind = d + 1;
+    out[kind] = outs[f];
+    a.store[kind] = 1;
+  }
+  a.src.P = P;
+  a.src.W = W;
+  a.src.n1 = d1.n;
+  a.src.n2 = d2.n;
+  a.src.ncp = static_cast<long long>(d1.n) * d2.n;
+  a.span1 = d1.span;
+  a.m1 = d1.m;
+  a.m2 = d2.m;
+  a.p1 = d1.p;
+  a.rows_per_wave = pl.fwd.rows_per_wave;
+  a.wx = pl.fwd.wx;
+  a.ntx = pl.fwd.n_tiles1;
+  a.nty = pl.fwd.n_chunks2;
+  a.xcd_map = pl.fwd.xcd_map;
+  a.nsamp = static_cast<long long>(d1.m) * d2.m;
+  // order-1 tables only where a derivative is asked for (D may be 0 elsewhere)
+  a.tab1[0] = order_table<T>(d1, 0);
+  a.tab1[1] = order_table<T>(d1, a.store[1] ? 1 : 0);
+  const T* tb2 = order_table<T>(d2, 0);
+  const T* tb2d = order_table<T>(d2, a.store[2] ? 1 : 0);
+  unsigned gz = 1;
+  const bool d3 = a.store[3] != 0;
+  if (pl.nin == 3) {
+    const DimDev& D3 = pl.dims[2];
+    a.src.span3 = D3.span;
+    a.src.p3 = D3.p;
+    a.src.tab3 = order_table<T>(D3, 0);
+    a.tab3d = order_table<T>(D3, d3 ? 1 : 0);
+    a.src.ncp *= D3.n;
+    a.nsamp *= D3.m;
+    gz = static_cast<unsigned>(D3.m);
+  }
+  const dim3 grid(pl.fwd.n_tiles1 * pl.fwd.n_chunks2 * gz);
+  for (int c0 = 0; c0 < pl.nout; c0 += nc_max) {
+    const int nc = std::min(nc_max, pl.nout - c0);
+    RatGradKernel<T> k = pl.nin == 2 ? pick_rat_grad_p2<T, 2, false>(d2.p, nc)
+                         : d3        ? pick_rat_grad_p2<T, 3, true>(d2.p, nc)
+                                     : pick_rat_grad_p2<T, 3, false>(d2.p, nc);
+    if (!k) return hipErrorInvalidValue;
+    a.c0 = c0;
+    a.src.c0 = c0;
+    const size_t lds = static_cast<size_t>(d3 ? 2 : 1) * (nc + 1) * pl.fwd.w1max *
+                       pl.fwd.w2max * sizeof(T);
+    hipLaunchKernelGGL(k, grid, dim3(kThreads), lds, s, a, d2.span, tb2, tb2d, out[0], out[1],
+                       out[2], out[3]);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
 
 template <typename T>
 __global__ void homogenize(const T* P, const T* W, T* H, long long ncp, int nout) {
@@ -189,6 +475,8 @@ long long binom(int n, int k) {
 template <typename T>
 hipError_t rational(const Plan& pl, int nf, const int32_t* orders, const T* P, const T* W,
                     T* const* outs, void* work, hipStream_t s) {
+  if (const int nc = rat_grad_channels(pl, nf, orders, sizeof(T)))
+    return rational_first_order(pl, nf, orders, P, W, outs, nc, s);
   const std::vector<Order> cl = closure_of(pl, nf, orders);
   const int K = static_cast<int>(cl.size());
   long long ncp = 1, nsamp = 1;

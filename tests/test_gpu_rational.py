@@ -23,6 +23,11 @@ CASES = [
     ((6, 7, 5), (2, 3, 2), (21, 18, 13), (1, 1, 1), 3,
      [[(1, 0, 0), (0, 1, 0), (0, 0, 1)], [(1, 1, 1)]]),
     ((12,), (3,), (60,), (2,), 2, [[(1,), (2,)]]),  # 1-D (direct kernel)
+    # value and first derivatives only: the fused single pass
+    ((9, 8), (3, 2), (37, 29), (1, 1), 3, [[(0, 0), (1, 0), (0, 1)], [(0, 1)]]),
+    ((6, 7, 5), (2, 3, 2), (21, 18, 13), (1, 1, 1), 2,
+     [[(0, 0, 0), (0, 0, 1)], [(1, 0, 0), (0, 1, 0), (0, 0, 1), (0, 0, 0)]]),
+    ((7, 6, 5), (3, 2, 1), (70, 20, 9), (1, 1, 1), 4, [[(1, 0, 0), (0, 0, 1)]]),
 ]
 
 
@@ -45,6 +50,27 @@ def test_rational_fields_vs_oracle(gpu, case, dtype, bar):
         for f, out in zip(fields, outs):
             want = O.rational_derivs(od, P.astype(np.float64), W.astype(np.float64), f)
             assert rel_err(eval_host(sg, out), want) < bar, (f, rel_err(eval_host(sg, out), want))
+
+
+@pytest.mark.parametrize("dtype,bar", [(np.float64, 1e-12), (np.float32, 1e-5)])
+def test_rational_fused_first_order_matches_two_pass(gpu, dtype, bar):
+    """d/dx1 and d/dx3 from the fused pass against the same fields from the
+    two-pass path (forced by adding a mixed second order to the request)."""
+    sg = gpu
+    n, p, m = (8, 9, 6), (3, 2, 2), (50, 31, 11)
+    dims = make_dims(sg, n, p, m, D=(1, 1, 1), dtype=dtype)
+    grid = sg.NURBSGrid(tuple(dims), 3)
+    rng = np.random.default_rng(11)
+    P = rng.uniform(-1, 1, tuple(n) + (3,)).astype(dtype)
+    W = rng.uniform(0.5, 2.0, n).astype(dtype)
+    grid.control_points.copy_(sg.from_numpy_colmajor(P, grid.dtype, "cuda"))
+    grid.weights.copy_(sg.from_numpy_colmajor(W, grid.dtype, "cuda"))
+    fused = [torch.empty_like(grid.eval) for _ in range(2)]
+    sg.evaluate_rational_fields_(grid, [(1, 0, 0), (0, 0, 1)], fused)
+    general = [torch.empty_like(grid.eval) for _ in range(3)]
+    sg.evaluate_rational_fields_(grid, [(1, 0, 0), (0, 0, 1), (1, 0, 1)], general)
+    for a, b in zip(fused, general[:2]):
+        assert rel_err(eval_host(sg, a), eval_host(sg, b)) < bar
 
 
 def test_rational_order_zero_is_evaluate(gpu):
