@@ -4,8 +4,9 @@
 // src/adjoint.jl:57).
 //
 // Adjoint: with the weights fixed R is linear in P, R = B (w P) / w(x), so
-// Pbar_k = w_k [B^T (ebar / w(x))]_k: the weight field by the B-spline
-// forward, a division, the B-spline adjoint, a product.
+// Pbar_k = w_k [B^T (ebar / w(x))]_k: ebar / w(x) in one streaming pass (the
+// weight field in registers, as the B-spline forward computes it), the
+// B-spline adjoint, a product.
 //
 // Two passes.  (1) The homogeneous control points (w P_c, w) go through the
 // B-spline forward (launch_forward on a view of the plan without weights and
@@ -383,7 +384,105 @@ void* combine_kernel_k(int k) {
   }
 }
 
-// ebar / w(x), channel by channel (the weight field w(x) in D).
+// ebar / w(x) for every channel in one pass: the forward's tiles over the
+// weights as a one-channel B-spline (stage 1 window of w, the register
+// window over k2), and each row's w(x) divides the row of ebar of every
+// channel as it is read.
+template <typename T>
+struct DivArgs {
+  BandSrc<T> src;  // P = the weights, one channel
+  const T* tab1;
+  const int32_t* span1;
+  const T* E;
+  T* out;
+  int m1, m2, p1, nout, rows_per_wave, wx, ntx, nty, xcd_map;
+  long long nsamp;
+};
+
+template <typename T, int NIN, int P2>
+__global__ __launch_bounds__(kThreads) void divide_by_weight_field(
+    DivArgs<T> a, const int32_t* __restrict__ span2, const T* __restrict__ tb2) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  T* G = reinterpret_cast<T*>(smem);
+  constexpr int PM = kMaxStreamDegree;
+  const int lane = threadIdx.x & (kLanes - 1);
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kLanes);
+  const int wy = kWaves / a.wx;
+  const FwdTile tile = fwd_tile(a.xcd_map, a.ntx, a.nty, kLanes * a.wx, a.rows_per_wave * wy,
+                                a.m1, a.m2, NIN);
+  const int j1_0 = tile.j1_0, r0 = tile.r0, r1 = tile.r1, j3 = tile.j3;
+  const int lo1 = a.span1[j1_0] - a.p1;
+  const int W1 = a.span1[tile.j1_last] - lo1 + 1;
+  const int lo2 = span2[r0] - P2;
+  const int W2 = span2[r1 - 1] - lo2 + 1;
+  stage1_band_window<T, NIN, 1, false>(G, a.src, j3, lo1, W1, lo2, W2);
+  __syncthreads();
+  const int wave_x = wave % a.wx, wave_y = wave / a.wx;
+  const int rb = r0 + wave_y * a.rows_per_wave;
+  const int re = min(rb + a.rows_per_wave, r1);
+  const int j1 = j1_0 + kLanes * wave_x + lane;
+  if (rb >= re || j1 >= a.m1) return;  // no barrier follows
+  const int p1 = a.p1;
+  const int q1b = a.span1[j1] - p1 - lo1;
+  T b1[PM + 1];
+#pragma unroll
+  for (int t = 0; t <= PM; ++t) {
+    const T v = a.tab1[static_cast<size_t>(j1) * (p1 + 1) + min(t, p1)];
+    b1[t] = t <= p1 ? v : T(0);
+  }
+  T V[P2 + 1];
+  auto column = [&](int k2) {
+    const T* g = G + (k2 - lo2) * W1 + q1b;
+    T s = T(0);
+#pragma unroll
+    for (int t = 0; t <= PM; ++t) s += b1[t] * g[min(t, p1)];
+    return s;
+  };
+  int cur = span2[rb];
+#pragma unroll
+  for (int q = 0; q <= P2; ++q) V[q] = column(cur - P2 + q);
+  long long o = static_cast<long long>(j1) + static_cast<long long>(a.m1) * a.m2 * j3 +
+                static_cast<long long>(a.m1) * rb;
+  for (int j2 = rb; j2 < re; ++j2, o += a.m1) {
+    const int s2 = span2[j2];
+    if (s2 != cur) {
+      if (s2 - cur > P2) {
+        cur = s2;
+#pragma unroll
+        for (int q = 0; q <= P2; ++q) V[q] = column(cur - P2 + q);
+      } else {
+        while (cur < s2) {
+#pragma unroll
+          for (int q = 0; q < P2; ++q) V[q] = V[q + 1];
+          ++cur;
+          V[P2] = column(cur);
+        }
+      }
+    }
+    const T* bb = tb2 + static_cast<size_t>(j2) * (P2 + 1);
+    T w = T(0);
+#pragma unroll
+    for (int q = 0; q <= P2; ++q) w += bb[q] * V[q];
+    const T inv = T(1) / w;
+    for (int c = 0; c < a.nout; ++c) a.out[o + a.nsamp * c] = a.E[o + a.nsamp * c] * inv;
+  }
+}
+
+template <typename T, int NIN>
+auto pick_divide(int p2) -> void (*)(DivArgs<T>, const int32_t*, const T*) {
+  switch (p2) {
+    case 0: return divide_by_weight_field<T, NIN, 0>;
+    case 1: return divide_by_weight_field<T, NIN, 1>;
+    case 2: return divide_by_weight_field<T, NIN, 2>;
+    case 3: return divide_by_weight_field<T, NIN, 3>;
+    case 4: return divide_by_weight_field<T, NIN, 4>;
+    case 5: return divide_by_weight_field<T, NIN, 5>;
+  }
+  return nullptr;
+}
+
+// ebar / w(x), channel by channel (the weight field w(x) in D): the general
+// case, after the weight field went through the B-spline forward.
 template <typename T>
 __global__ void divide_by_field(const T* E, const T* D, T* out, long long nsamp, int nout) {
   const long long i = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
@@ -391,6 +490,53 @@ __global__ void divide_by_field(const T* E, const T* D, T* out, long long nsamp,
   const double inv = 1.0 / static_cast<double>(D[i]);
   for (int c = 0; c < nout; ++c)
     out[i + nsamp * c] = static_cast<T>(static_cast<double>(E[i + nsamp * c]) * inv);
+}
+
+// The one-pass division: streaming grids (2-D, 3-D) whose weight window
+// fits; false otherwise.
+template <typename T>
+bool divide_streaming(const Plan& pl, const T* W, const T* E, T* out, hipStream_t s,
+                      hipError_t& e) {
+  if (!pl.stream_ok || (pl.nin != 2 && pl.nin != 3)) return false;
+  const size_t lds = static_cast<size_t>(pl.fwd.w1max) * pl.fwd.w2max * sizeof(T);
+  if (lds > static_cast<size_t>(kMaxLds)) return false;
+  const DimDev& d1 = pl.dims[0];
+  const DimDev& d2 = pl.dims[1];
+  DivArgs<T> a{};
+  a.src.P = W;
+  a.src.n1 = d1.n;
+  a.src.n2 = d2.n;
+  a.src.ncp = static_cast<long long>(d1.n) * d2.n;
+  a.tab1 = order_table<T>(d1, 0);
+  a.span1 = d1.span;
+  a.E = E;
+  a.out = out;
+  a.m1 = d1.m;
+  a.m2 = d2.m;
+  a.p1 = d1.p;
+  a.nout = pl.nout;
+  a.rows_per_wave = pl.fwd.rows_per_wave;
+  a.wx = pl.fwd.wx;
+  a.ntx = pl.fwd.n_tiles1;
+  a.nty = pl.fwd.n_chunks2;
+  a.xcd_map = pl.fwd.xcd_map;
+  a.nsamp = static_cast<long long>(d1.m) * d2.m;
+  unsigned gz = 1;
+  if (pl.nin == 3) {
+    const DimDev& d3 = pl.dims[2];
+    a.src.span3 = d3.span;
+    a.src.p3 = d3.p;
+    a.src.tab3 = order_table<T>(d3, 0);
+    a.src.ncp *= d3.n;
+    a.nsamp *= d3.m;
+    gz = static_cast<unsigned>(d3.m);
+  }
+  auto k = pl.nin == 2 ? pick_divide<T, 2>(d2.p) : pick_divide<T, 3>(d2.p);
+  if (!k) return false;
+  hipLaunchKernelGGL(k, dim3(pl.fwd.n_tiles1 * pl.fwd.n_chunks2 * gz), dim3(kThreads), lds, s, a,
+                     d2.span, order_table<T>(d2, 0));
+  e = hipGetLastError();
+  return true;
 }
 
 // Pbar = (accumulate ? Pbar : 0) + w * Q, channel by channel.
@@ -420,17 +566,19 @@ hipError_t rational_adjoint(const Plan& pl, const T* W, const T* E, T* Pbar, voi
   T* Q = Et + nsamp * pl.nout;
   const int dtype = sizeof(T) == 4 ? SG_F32 : SG_F64;
   const int32_t zero[SG_MAX_NIN] = {0, 0, 0, 0, 0, 0, 0, 0};
-  Plan wp = pl;  // a view: the weights as a one-channel B-spline
-  wp.nurbs = 0;
-  wp.nout = 1;
-  void* dout[1] = {D};
-  hipError_t e = launch_forward(wp, 1, zero, W, nullptr, dout, dtype, s);
-  if (e != hipSuccess) return e;
-  if (nsamp > 0) {
+  hipError_t e = hipSuccess;
+  if (nsamp > 0 && !divide_streaming(pl, W, E, Et, s, e)) {
+    Plan wp = pl;  // a view: the weights as a one-channel B-spline
+    wp.nurbs = 0;
+    wp.nout = 1;
+    void* dout[1] = {D};
+    e = launch_forward(wp, 1, zero, W, nullptr, dout, dtype, s);
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(divide_by_field<T>, dim3(static_cast<unsigned>((nsamp + 255) / 256)),
                        dim3(256), 0, s, E, D, Et, nsamp, pl.nout);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
+    e = hipGetLastError();
   }
+  if (e != hipSuccess) return e;
   Plan bp = pl;  // a view: the B-spline adjoint of the same dimensions
   bp.nurbs = 0;
   e = launch_adjoint(bp, zero, Et, Q, dtype, 0, s);
