@@ -3,9 +3,13 @@
 // contiguous fill with 4-, 8- and 16-byte stores per lane, and (b) in the
 // forward kernel's own pattern: tiles of 64 samples j1 (256 contiguous
 // bytes of a row) x rows j2 x one sample j3, every row storing the three
-// channel planes, with nothing computed.  (b) is the ceiling the forward
-// kernel's store stream can reach; bench.py --workload ceiling runs this
-// program and prints its lines.  Output: one JSON object per measurement.
+// channel planes, with nothing computed, with workgroups numbered as the
+// forward numbers them by default (j1 tiles fastest, then chunks, then j3,
+// dealt round-robin over the XCDs) and (c) with each XCD taking one
+// contiguous eighth of the tiles (the forward's XCD map 1).  (b) is the
+// ceiling the forward kernel's store stream can reach; bench.py --workload
+// ceiling runs this program and prints its lines.  Output: one JSON object
+// per measurement.
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -28,11 +32,16 @@ __global__ __launch_bounds__(256) void fill(float* __restrict__ out) {
 }
 
 // the forward's tiles: 64 lanes along j1, 4 waves of `rows` rows along j2,
-// one sample j3; consecutive workgroups walk j1 tiles, then chunks, then j3
-__global__ __launch_bounds__(256) void pattern(float* __restrict__ out, int rows) {
+// one sample j3; consecutive tiles walk j1 tiles, then chunks, then j3; with
+// xcd_contiguous, workgroup b (on XCD b mod 8) takes tile (b mod 8) n / 8 +
+// b / 8 of the n tiles (n is a multiple of 8 here)
+__global__ __launch_bounds__(256) void pattern(float* __restrict__ out, int rows,
+                                               int xcd_contiguous) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int ntx = M1 / 64, nty = M2 / (rows * 4);
-  const int L = blockIdx.x;
+  const int n = static_cast<int>(gridDim.x);
+  const int b = blockIdx.x;
+  const int L = xcd_contiguous ? (b % 8) * (n / 8) + b / 8 : b;
   const int bx = L % ntx, by = (L / ntx) % nty, j3 = L / (ntx * nty);
   const int r0 = by * rows * 4 + wave * rows;
   const long long plane = static_cast<long long>(M1) * M2 * M3;
@@ -73,12 +82,14 @@ int main() {
   timeit([&] { hipLaunchKernelGGL(fill<1>, dim3(b1), dim3(256), 0, 0, d); }, "fill 4 B per lane");
   timeit([&] { hipLaunchKernelGGL(fill<2>, dim3(b2), dim3(256), 0, 0, d); }, "fill 8 B per lane");
   timeit([&] { hipLaunchKernelGGL(fill<4>, dim3(b4), dim3(256), 0, 0, d); }, "fill 16 B per lane");
-  for (int rows : {16, 32}) {
-    const unsigned grid = (M1 / 64) * (M2 / (rows * 4)) * M3;
-    char name[64];
-    std::snprintf(name, sizeof name, "forward pattern, %d rows per wave", rows);
-    timeit([&] { hipLaunchKernelGGL(pattern, dim3(grid), dim3(256), 0, 0, d, rows); }, name);
-  }
+  for (int xc : {0, 1})
+    for (int rows : {8, 16, 32}) {
+      const unsigned grid = (M1 / 64) * (M2 / (rows * 4)) * M3;
+      char name[96];
+      std::snprintf(name, sizeof name, "forward pattern, %d rows per wave%s", rows,
+                    xc ? ", XCD-contiguous tiles" : "");
+      timeit([&] { hipLaunchKernelGGL(pattern, dim3(grid), dim3(256), 0, 0, d, rows, xc); }, name);
+    }
   (void)hipFree(d);
   return failures ? 1 : 0;
 }
