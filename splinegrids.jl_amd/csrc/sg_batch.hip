@@ -38,6 +38,8 @@ struct BGrid {
   int m1, m2, m3;
   int rows_per_wave;
   int list_stride;
+  int vec;  // samples j1 per lane (batch_vec)
+  int pad;
   long long ncp, nsamp;
 };
 
@@ -73,6 +75,10 @@ struct TileCtx {
   bool active;
   int j1;
   float b1[kMaxStreamDegree + 1];
+  // tiles of two samples per lane: the lane's second sample j1b = j1 + 1
+  bool activeb;
+  int j1b, q1bb;
+  float b1b[kMaxStreamDegree + 1];
   // the wave's rows rb .. re - 1 (at most kRowSegs * kLanes of them): lane l
   // holds the span and B2 row of row rb + l in segment 0 and of row
   // rb + kLanes + l in segment 1; the row loop reads them with readlane, so
@@ -194,8 +200,9 @@ __device__ __attribute__((always_inline)) void stage1_3d(const TileCtx& x, int c
 // plane in LDS), then each wave's rows.  P1, P2, NC and NURBS are
 // compile-time, so a window column is exactly P1 + 1 LDS reads and FMAs per
 // channel, and the row loop does exactly NC (+ weight) window sums and stores.
-template <int P1, int P2, int NC, bool NURBS>
+template <int P1, int P2, int NC, bool NURBS, int VEC>
 __device__ __attribute__((always_inline)) void tile_pass(const TileCtx& x, int c0) {
+  typedef float f2u __attribute__((ext_vector_type(2), aligned(4)));
   constexpr int NCH = NC + (NURBS ? 1 : 0);
   const int cs = x.cs;
   // ---- stage 1: band windows, contracted along dimension 3 ----
@@ -212,22 +219,28 @@ __device__ __attribute__((always_inline)) void tile_pass(const TileCtx& x, int c
   if (x.rb < x.re) {
     // 3-D tiles: each lane reads its plane's slab; 2-D tiles: the one plane
     const float* G = x.smem + (x.list != nullptr ? static_cast<size_t>(x.pz) * x.nch * cs : 0);
-    float V[P2 + 1][NCH];
-    // all (P1 + 1) * NCH window reads of a column in flight before the sums
-    auto column = [&](int k2, float (&v)[NCH]) {
-      const float* gr = G + (k2 - x.lo2) * x.W1 + x.q1b;
-      float t[NCH][P1 + 1];
+    // V[q][h][ch]: window column k2 = cur - P2 + q of the lane's sample h
+    float V[P2 + 1][VEC][NCH];
+    // all (P1 + 1) * NCH * VEC window reads of a column in flight before the sums
+    auto column = [&](int k2, float (&v)[VEC][NCH]) {
+      float t[VEC][NCH][P1 + 1];
 #pragma unroll
-      for (int ch = 0; ch < NCH; ++ch)
+      for (int h = 0; h < VEC; ++h) {
+        const float* gr = G + (k2 - x.lo2) * x.W1 + (h == 0 ? x.q1b : x.q1bb);
 #pragma unroll
-        for (int q = 0; q <= P1; ++q) t[ch][q] = gr[ch * cs + q];
+        for (int ch = 0; ch < NCH; ++ch)
 #pragma unroll
-      for (int ch = 0; ch < NCH; ++ch) {
-        float s = 0.f;
-#pragma unroll
-        for (int q = 0; q <= P1; ++q) s += x.b1[q] * t[ch][q];
-        v[ch] = s;
+          for (int q = 0; q <= P1; ++q) t[h][ch][q] = gr[ch * cs + q];
       }
+#pragma unroll
+      for (int h = 0; h < VEC; ++h)
+#pragma unroll
+        for (int ch = 0; ch < NCH; ++ch) {
+          float s = 0.f;
+#pragma unroll
+          for (int q = 0; q <= P1; ++q) s += (h == 0 ? x.b1[q] : x.b1b[q]) * t[h][ch][q];
+          v[h][ch] = s;
+        }
     };
     int cur = __builtin_amdgcn_readlane(x.rspan[0], 0);
 #pragma unroll
@@ -257,24 +270,43 @@ __device__ __attribute__((always_inline)) void tile_pass(const TileCtx& x, int c
 #pragma unroll
             for (int q = 0; q < P2; ++q)
 #pragma unroll
-              for (int ch = 0; ch < NCH; ++ch) V[q][ch] = V[q + 1][ch];
+              for (int h = 0; h < VEC; ++h)
+#pragma unroll
+                for (int ch = 0; ch < NCH; ++ch) V[q][h][ch] = V[q + 1][h][ch];
             ++cur;
             column(cur, V[P2]);
           }
         }
       }
-      float num[NCH];
+      float val[VEC][NC];
 #pragma unroll
-      for (int ch = 0; ch < NCH; ++ch) {
-        float s = 0.f;
+      for (int h = 0; h < VEC; ++h) {
+        float num[NCH];
 #pragma unroll
-        for (int q = 0; q <= P2; ++q) s += bb[q] * V[q][ch];
-        num[ch] = s;
+        for (int ch = 0; ch < NCH; ++ch) {
+          float s = 0.f;
+#pragma unroll
+          for (int q = 0; q <= P2; ++q) s += bb[q] * V[q][h][ch];
+          num[ch] = s;
+        }
+        const float inv = NURBS ? nurbs_recip(num[NC]) : 1.f;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) val[h][c] = NURBS ? num[c] * inv : num[c];
       }
-      const float inv = NURBS ? nurbs_recip(num[NC]) : 1.f;
-      if (x.active) {
+      if constexpr (VEC == 2) {
+        // both samples: one 8-byte store per channel (4-byte aligned, as
+        // global stores allow); the tile's last odd sample alone: 4 bytes
+        if (x.activeb) {
 #pragma unroll
-        for (int c = 0; c < NC; ++c) x.out[o + x.nsamp * c] = NURBS ? num[c] * inv : num[c];
+          for (int c = 0; c < NC; ++c)
+            *reinterpret_cast<Global<f2u>*>(x.out + o + x.nsamp * c) = f2u{val[0][c], val[1][c]};
+        } else if (x.active) {
+#pragma unroll
+          for (int c = 0; c < NC; ++c) x.out[o + x.nsamp * c] = val[0][c];
+        }
+      } else if (x.active) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) x.out[o + x.nsamp * c] = val[0][c];
       }
       o += x.m1;
     }
@@ -283,25 +315,31 @@ __device__ __attribute__((always_inline)) void tile_pass(const TileCtx& x, int c
   __syncthreads();  // G is rewritten by the next channel block / tile
 }
 
-template <int P1, int P2, bool NURBS>
+template <int P1, int P2, bool NURBS, int VEC>
 __device__ __attribute__((always_inline)) void tile_pass_nc(const TileCtx& x, int c0, int nc) {
   switch (nc) {
-    case 1: tile_pass<P1, P2, 1, NURBS>(x, c0); break;
-    case 2: tile_pass<P1, P2, 2, NURBS>(x, c0); break;
-    case 3: tile_pass<P1, P2, 3, NURBS>(x, c0); break;
-    default: tile_pass<P1, P2, 4, NURBS>(x, c0); break;
+    case 1: tile_pass<P1, P2, 1, NURBS, VEC>(x, c0); break;
+    case 2: tile_pass<P1, P2, 2, NURBS, VEC>(x, c0); break;
+    case 3: tile_pass<P1, P2, 3, NURBS, VEC>(x, c0); break;
+    default:
+      // two samples per lane: blocks of at most kVec2Slabs slabs
+      if constexpr (VEC == 1 || !NURBS) tile_pass<P1, P2, 4, NURBS, VEC>(x, c0);
+      break;
   }
 }
 
-template <int P1, int P2>
+// VEC samples j1 per lane: tiles of kLanes * VEC samples (VEC = 2: 3-D grids
+// whose rows fit one tile, which then stores each row in whole 8-byte pairs)
+template <int P1, int P2, int VEC>
 __device__ __attribute__((always_inline)) void batch_tile(const BGrid& g, const BTile& t,
                                                           float* smem, int lane, int wave) {
   // the grid's arrays as global pointers (global loads and stores)
   const auto span1 = global_ptr(g.span1), span2 = global_ptr(g.span2);
   const auto tab1 = global_ptr(g.tab1);
   const int chunk_cap = g.rows_per_wave * kWaves;
-  const int j1_0 = t.bx * kLanes;
-  const int j1_last = min(j1_0 + kLanes - 1, g.m1 - 1);
+  constexpr int tw = kLanes * VEC;  // samples j1 per tile
+  const int j1_0 = t.bx * tw;
+  const int j1_last = min(j1_0 + tw - 1, g.m1 - 1);
   const int r0 = t.by * chunk_cap, r1 = min(r0 + chunk_cap, g.m2);
   constexpr int p1 = P1;
   TileCtx x;
@@ -316,9 +354,10 @@ __device__ __attribute__((always_inline)) void batch_tile(const BGrid& g, const 
   x.nch = min(kBC, g.nout) + (nurbs ? 1 : 0);
   // 3-D grids: the tile holds kWaves consecutive samples j3 (from
   // kWaves * t.bz), one per wave, and every wave walks all rows of the chunk;
-  // a tile 32 or 16 samples wide packs 2 or 4 planes into the lane groups of
-  // a wave, which then walks a half or a quarter of the rows (the same rows
-  // in every group, so spans and B2 rows stay uniform over the wave);
+  // a tile 32 or 16 samples wide (one sample per lane) packs 2 or 4 planes
+  // into the lane groups of a wave, which then walks a half or a quarter of
+  // the rows (the same rows in every group, so spans and B2 rows stay uniform
+  // over the wave);
   // 2-D grids: one plane, the waves split the chunk's rows
   const bool zpack = g.nin == 3;
   x.j3_0 = zpack ? t.bz * kWaves : 0;
@@ -331,8 +370,8 @@ __device__ __attribute__((always_inline)) void batch_tile(const BGrid& g, const 
   x.out = global_ptr(g.out);
   x.n1 = g.n1;
   x.p3 = g.p3;
-  const int width = min(kLanes, g.m1 - j1_0);  // samples j1 of the tile
-  const int groups = zpack ? (width <= 16 ? 4 : width <= 32 ? 2 : 1) : 1;
+  const int width = min(tw, g.m1 - j1_0);  // samples j1 of the tile
+  const int groups = zpack && VEC == 1 ? (width <= 16 ? 4 : width <= 32 ? 2 : 1) : 1;
   const int group_lanes = kLanes / groups;
   const int grp = lane / group_lanes, li = lane - grp * group_lanes;
   const int slot0 = wave * groups;  // the wave's first plane slot
@@ -368,32 +407,57 @@ __device__ __attribute__((always_inline)) void batch_tile(const BGrid& g, const 
   x.ncp = g.ncp;
   x.nsamp = g.nsamp;
   x.plane_cp = static_cast<long long>(g.n1) * g.n2;
-  x.active = zpack ? (li < width && x.pz < x.planes) : (j1_0 + lane) < g.m1;
-  x.j1 = x.active ? j1_0 + (zpack ? li : lane) : g.m1 - 1;
+  // lanes outside the tile compute on the tile's last sample (in the window)
+  // and do not store
+  const bool plane_ok = !zpack || x.pz < x.planes;
+  x.active = VEC * li < width && plane_ok;
+  x.j1 = x.active ? j1_0 + VEC * li : j1_last;
   x.q1b = span1[x.j1] - p1 - x.lo1;
+  x.activeb = VEC == 2 && VEC * li + 1 < width && plane_ok;
+  x.j1b = x.activeb ? x.j1 + 1 : x.j1;
+  x.q1bb = VEC == 2 ? span1[x.j1b] - p1 - x.lo1 : x.q1b;
   x.list = zpack ? constant_ptr(g.lists) + static_cast<size_t>(t.bz) * g.list_stride : nullptr;
   x.nk = zpack ? reinterpret_cast<Constant<const int>*>(x.list)[kEntryN] : 0;
 #pragma unroll
   for (int q = 0; q <= P1; ++q) x.b1[q] = tab1[static_cast<size_t>(x.j1) * (P1 + 1) + q];
-  for (int c0 = 0; c0 < g.nout; c0 += kBC) {
-    const int nc = min(kBC, g.nout - c0);
-    if (nurbs)
-      tile_pass_nc<P1, P2, true>(x, c0, nc);
-    else
-      tile_pass_nc<P1, P2, false>(x, c0, nc);
+  if constexpr (VEC == 2) {
+#pragma unroll
+    for (int q = 0; q <= P1; ++q) x.b1b[q] = tab1[static_cast<size_t>(x.j1b) * (P1 + 1) + q];
   }
+  // two samples per lane: at most kVec2Slabs slabs per block (registers)
+  const int bc = VEC == 2 ? kVec2Slabs - (nurbs ? 1 : 0) : kBC;
+  for (int c0 = 0; c0 < g.nout; c0 += bc) {
+    const int nc = min(bc, g.nout - c0);
+    if (nurbs)
+      tile_pass_nc<P1, P2, true, VEC>(x, c0, nc);
+    else
+      tile_pass_nc<P1, P2, false, VEC>(x, c0, nc);
+  }
+}
+
+template <int P1, int P2>
+__device__ __attribute__((always_inline)) void batch_tile_v(const BGrid& g, const BTile& t,
+                                                            float* smem, int lane, int wave) {
+  // the host gives two samples per lane only to degrees <= kVec2MaxDegree
+  if constexpr (P1 <= kVec2MaxDegree && P2 <= kVec2MaxDegree) {
+    if (g.vec == 2) {
+      batch_tile<P1, P2, 2>(g, t, smem, lane, wave);
+      return;
+    }
+  }
+  batch_tile<P1, P2, 1>(g, t, smem, lane, wave);
 }
 
 template <int P1>
 __device__ __attribute__((always_inline)) void batch_tile_p2(const BGrid& g, const BTile& t,
                                                              float* smem, int lane, int wave) {
   switch (g.p2) {
-    case 0: batch_tile<P1, 0>(g, t, smem, lane, wave); break;
-    case 1: batch_tile<P1, 1>(g, t, smem, lane, wave); break;
-    case 2: batch_tile<P1, 2>(g, t, smem, lane, wave); break;
-    case 3: batch_tile<P1, 3>(g, t, smem, lane, wave); break;
-    case 4: batch_tile<P1, 4>(g, t, smem, lane, wave); break;
-    default: batch_tile<P1, 5>(g, t, smem, lane, wave); break;
+    case 0: batch_tile_v<P1, 0>(g, t, smem, lane, wave); break;
+    case 1: batch_tile_v<P1, 1>(g, t, smem, lane, wave); break;
+    case 2: batch_tile_v<P1, 2>(g, t, smem, lane, wave); break;
+    case 3: batch_tile_v<P1, 3>(g, t, smem, lane, wave); break;
+    case 4: batch_tile_v<P1, 4>(g, t, smem, lane, wave); break;
+    default: batch_tile_v<P1, 5>(g, t, smem, lane, wave); break;
   }
 }
 
@@ -628,9 +692,8 @@ int build_tiles(sg_batch* b, int rows_per_wave, int blocks_per_cu) {
   b->separate = b->fixed_separate;
   for (int gi : b->candidates) {
     const Plan& pl = *b->plans[gi];
-    int rows = rows_per_wave;
-    GridTiling t = grid_tiling(pl, rows);
-    while (t.lds > cap && rows > 1) t = grid_tiling(pl, rows /= 2);
+    const GridTiling t = batch_grid_tiling(pl, rows_per_wave, cap);
+    const int rows = t.rows;
     if (t.lds > cap) {
       b->separate.push_back(gi);
       continue;
@@ -649,7 +712,7 @@ int build_tiles(sg_batch* b, int rows_per_wave, int blocks_per_cu) {
       for (int y = 0; y < t.nty; ++y)
         for (int x = 0; x < t.ntx; ++x) {
           tiles.push_back(BTile{gi, x, y, z});
-          const long long lanes = std::min(kLanes, d1.m - x * kLanes);
+          const long long lanes = std::min(t.tw, d1.m - x * t.tw);
           const long long nrows = std::min(chunk, d2.m - y * chunk);
           const long long planes = pl.nin == 3 ? std::min(kWaves, m3 - z * kWaves) : 1;
           cost += lanes * nrows * planes * pl.nout + 64LL * planes * t.w1 * t.w2;
@@ -657,6 +720,7 @@ int build_tiles(sg_batch* b, int rows_per_wave, int blocks_per_cu) {
     grid_tiles.push_back(std::move(tiles));
     grid_cost.push_back(cost);
     b->hgrids[gi].rows_per_wave = rows;
+    b->hgrids[gi].vec = t.tw / kLanes;
   }
   std::vector<BTile> sorted;
   int host_q[kQueueInts] = {};

@@ -288,14 +288,23 @@ size_t lds_cap(int blocks_per_cu) {
   return std::min(static_cast<size_t>(kMaxLds), (share & ~size_t(511)) - 512);
 }
 
-GridTiling grid_tiling(const Plan& pl, int rows) {
+int batch_vec(const Plan& pl) {
+  const int m1 = pl.dims[0].m;
+  return pl.nin == 3 && m1 > kLanes && m1 <= 2 * kLanes && pl.dims[0].p <= kVec2MaxDegree &&
+                 pl.dims[1].p <= kVec2MaxDegree
+             ? 2
+             : 1;
+}
+
+GridTiling grid_tiling(const Plan& pl, int rows, int vec) {
   const DimDev& d1 = pl.dims[0];
   const DimDev& d2 = pl.dims[1];
-  GridTiling t{rows, (d1.m + kLanes - 1) / kLanes, 0, 0, 0, 0};
+  const int tw = kLanes * vec;
+  GridTiling t{rows, (d1.m + tw - 1) / tw, 0, 0, 0, 0, tw};
   const int chunk = rows * kWaves;
   t.nty = (d2.m + chunk - 1) / chunk;
   for (int i = 0; i < t.ntx; ++i) {
-    const int a = i * kLanes, e = std::min(a + kLanes, d1.m) - 1;
+    const int a = i * tw, e = std::min(a + tw, d1.m) - 1;
     t.w1 = std::max(t.w1, d1.h_span[e] - d1.h_span[a] + d1.p + 1);
   }
   for (int c = 0; c < t.nty; ++c) {
@@ -306,6 +315,19 @@ GridTiling grid_tiling(const Plan& pl, int rows) {
   const size_t planes = pl.nin == 3 ? kWaves : 1;
   t.lds = planes * nch * t.w1 * t.w2 * sizeof(float);  // the window slabs
   return t;
+}
+
+GridTiling batch_grid_tiling(const Plan& pl, int rows_per_wave, size_t cap) {
+  auto fit = [&](int vec) {
+    int rows = rows_per_wave;
+    GridTiling t = grid_tiling(pl, rows, vec);
+    while (t.lds > cap && rows > 1) t = grid_tiling(pl, rows /= 2, vec);
+    return t;
+  };
+  const GridTiling t1 = fit(1);
+  if (batch_vec(pl) != 2) return t1;
+  const GridTiling t2 = fit(2);
+  return t2.lds <= cap ? t2 : t1;
 }
 
 int plane_list_host(const int32_t* span3, const float* tab3, int p3, int j3_0, int planes,

@@ -85,12 +85,29 @@ int plane_list_host(const int32_t* span3, const float* tab3, int p3, int j3_0, i
 // LDS one workgroup may take when blocks_per_cu of them share a CU.
 size_t lds_cap(int blocks_per_cu);
 
-// Window sizes of a grid's batch tiles for a rows-per-wave choice.
+// Samples j1 per lane a grid's batch tiles may take: 2 for 3-D grids of
+// degrees p1, p2 <= kVec2MaxDegree whose rows are longer than one wave and
+// fit two samples per lane (one tile per row, stored in 8-byte pairs), else
+// 1.  Such tiles take channel blocks of at most kVec2Slabs window slabs.
+constexpr int kVec2MaxDegree = 4;
+constexpr int kVec2Slabs = 4;
+int batch_vec(const Plan& pl);
+
+// Window sizes of a grid's batch tiles (kLanes * vec samples j1) for a
+// rows-per-wave choice.
 struct GridTiling {
   int rows, ntx, nty, w1, w2;
   size_t lds;  // bytes of LDS one tile of this grid needs
+  int tw;      // samples j1 per tile
 };
-GridTiling grid_tiling(const Plan& pl, int rows);
+GridTiling grid_tiling(const Plan& pl, int rows, int vec = 1);
+
+// The tiling of a grid in a batch of `rows_per_wave` under an LDS cap: rows
+// halve from rows_per_wave until the window fits; two samples per lane
+// where batch_vec allows them and their wider window fits at some number of
+// rows (on C5 that beats one sample per lane even with fewer rows), else
+// one.  lds > cap when nothing fits (the grid then needs its own launch).
+GridTiling batch_grid_tiling(const Plan& pl, int rows_per_wave, size_t cap);
 
 // The queues of a tile list: grids go whole to the least loaded queue,
 // largest first; a queue holds its grids in that order.  Writes the tiles in

@@ -271,9 +271,26 @@ void check_batch(std::vector<Plan>& plans) {
       if (!pl.stream_ok) continue;
       const GridTiling t = grid_tiling(pl, rows);
       const std::string nm = name_of(pl) + " batch rows " + std::to_string(rows);
-      check_windows(pl.dims[0], t.ntx, kLanes, t.w1, nm + " dim 1");
+      check_windows(pl.dims[0], t.ntx, t.tw, t.w1, nm + " dim 1");
       check_windows(pl.dims[1], t.nty, rows * kWaves, t.w2, nm + " dim 2");
       require(t.lds > 0, nm + ": LDS");
+      // the choice of rows and samples per lane under each LDS cap
+      for (int bpc : {2, 3, 4, 6}) {
+        const size_t cap = lds_cap(bpc);
+        const GridTiling c = batch_grid_tiling(pl, rows, cap);
+        const std::string cn = nm + " cap " + std::to_string(cap);
+        require(c.tw == kLanes || (c.tw == 2 * kLanes && batch_vec(pl) == 2), cn + ": samples per lane");
+        if (c.tw > kLanes)
+          require(c.ntx == 1 && pl.nin == 3 && pl.dims[0].p <= kVec2MaxDegree &&
+                      pl.dims[1].p <= kVec2MaxDegree && c.lds <= cap,
+                  cn + ": two samples per lane only where they fit, one tile per row");
+        require(c.rows >= 1 && c.rows <= rows && rows % c.rows == 0, cn + ": rows");
+        const GridTiling again = grid_tiling(pl, c.rows, c.tw / kLanes);
+        require(again.lds == c.lds && again.w1 == c.w1 && again.w2 == c.w2, cn + ": consistent");
+        check_windows(pl.dims[0], c.ntx, c.tw, c.w1, cn + " dim 1");
+        check_windows(pl.dims[1], c.nty, c.rows * kWaves, c.w2, cn + " dim 2");
+        if (c.lds > cap) require(grid_tiling(pl, 1, 1).lds > cap, cn + ": a fitting choice was missed");
+      }
       std::vector<BTile> tiles;
       const int nz = pl.nin == 3 ? (pl.dims[2].m + kWaves - 1) / kWaves : 1;
       for (int z = 0; z < nz; ++z)
