@@ -328,8 +328,9 @@ __device__ __attribute__((always_inline)) void tile_pass_nc(const TileCtx& x, in
   }
 }
 
-// VEC samples j1 per lane: tiles of kLanes * VEC samples (VEC = 2: 3-D grids
-// whose rows fit one tile, which then stores each row in whole 8-byte pairs)
+// VEC samples j1 per lane: tiles of kLanes * VEC samples (VEC = 2: grids whose
+// rows are longer than one wave; a tile stores its part of a row in 8-byte
+// pairs)
 template <int P1, int P2, int VEC>
 __device__ __attribute__((always_inline)) void batch_tile(const BGrid& g, const BTile& t,
                                                           float* smem, int lane, int wave) {

@@ -290,7 +290,7 @@ size_t lds_cap(int blocks_per_cu) {
 
 int batch_vec(const Plan& pl) {
   const int m1 = pl.dims[0].m;
-  return pl.nin == 3 && m1 > kLanes && m1 <= 2 * kLanes && pl.dims[0].p <= kVec2MaxDegree &&
+  return m1 > kLanes && pl.dims[0].p <= kVec2MaxDegree &&
                  pl.dims[1].p <= kVec2MaxDegree
              ? 2
              : 1;

@@ -85,10 +85,11 @@ int plane_list_host(const int32_t* span3, const float* tab3, int p3, int j3_0, i
 // LDS one workgroup may take when blocks_per_cu of them share a CU.
 size_t lds_cap(int blocks_per_cu);
 
-// Samples j1 per lane a grid's batch tiles may take: 2 for 3-D grids of
-// degrees p1, p2 <= kVec2MaxDegree whose rows are longer than one wave and
-// fit two samples per lane (one tile per row, stored in 8-byte pairs), else
-// 1.  Such tiles take channel blocks of at most kVec2Slabs window slabs.
+// Samples j1 per lane a grid's batch tiles may take: 2 for grids of degrees
+// p1, p2 <= kVec2MaxDegree whose rows are longer than one wave (tiles of
+// 2 kLanes samples, rows stored in 8-byte pairs: half the store
+// instructions, and a row of up to 128 samples in one tile), else 1.  Such
+// tiles take channel blocks of at most kVec2Slabs window slabs.
 constexpr int kVec2MaxDegree = 4;
 constexpr int kVec2Slabs = 4;
 int batch_vec(const Plan& pl);

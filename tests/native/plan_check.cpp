@@ -281,9 +281,9 @@ void check_batch(std::vector<Plan>& plans) {
         const std::string cn = nm + " cap " + std::to_string(cap);
         require(c.tw == kLanes || (c.tw == 2 * kLanes && batch_vec(pl) == 2), cn + ": samples per lane");
         if (c.tw > kLanes)
-          require(c.ntx == 1 && pl.nin == 3 && pl.dims[0].p <= kVec2MaxDegree &&
+          require(pl.dims[0].m > kLanes && pl.dims[0].p <= kVec2MaxDegree &&
                       pl.dims[1].p <= kVec2MaxDegree && c.lds <= cap,
-                  cn + ": two samples per lane only where they fit, one tile per row");
+                  cn + ": two samples per lane only for long rows and where they fit");
         require(c.rows >= 1 && c.rows <= rows && rows % c.rows == 0, cn + ": rows");
         const GridTiling again = grid_tiling(pl, c.rows, c.tw / kLanes);
         require(again.lds == c.lds && again.w1 == c.w1 && again.w2 == c.w2, cn + ": consistent");

@@ -32,7 +32,7 @@ def _batch_grids(sg, rng, n=40):
         dict(n=(9, 12, 7), p=(1, 2, 0), m=(20, 17, 7), nout=3, nurbs=True),
         dict(n=(8, 10, 40), p=(4, 1, 5), m=(30, 22, 13), nout=5, nurbs=True),
         dict(n=(8, 8), p=(3, 3), m=(65, 40), nout=3, nurbs=True, unsorted=True),
-        # 3-D rows of 65..128 samples: one tile per row, two samples per lane
+        # rows of 65..128 samples: one tile per row, two samples per lane
         # (odd rows end on a single sample; NURBS in blocks of three channels;
         # five channels in blocks of four); degree 5: one sample per lane
         dict(n=(20, 9, 6), p=(3, 2, 2), m=(65, 17, 9), nout=3, nurbs=False),
@@ -40,6 +40,8 @@ def _batch_grids(sg, rng, n=40):
         dict(n=(25, 7, 5), p=(2, 1, 1), m=(101, 10, 5), nout=1, nurbs=True),
         dict(n=(16, 6, 5), p=(5, 2, 2), m=(90, 9, 5), nout=2, nurbs=False),
         dict(n=(40, 10, 6), p=(1, 3, 0), m=(77, 19, 7), nout=5, nurbs=False),
+        # 2-D rows of 301 samples: three tiles of 128 (the last 45 wide, odd)
+        dict(n=(30, 20), p=(3, 4), m=(301, 45), nout=2, nurbs=True),
     ]
     return [bench.build_spec_grid(sg, spec, rng) for spec in grids]
 
