@@ -68,7 +68,7 @@ struct TileCtx {
   // null for 2-D tiles
   Constant<const float>* list;
   int nk;             // ... and its length
-  int n1, p3, nch, cs, W1, lo1, lo2, rb, re, j3_0, j3, planes, q1b, m1, m2;
+  int n1, nch, cs, W1, lo1, lo2, rb, re, j3_0, j3, planes, q1b, m1, m2;
   int pz;  // 3-D tiles: this lane's plane of the tile (its window slab)
   unsigned cs_rcp, w1_rcp;  // udiv_rcp of cs and W1
   long long ncp, nsamp, plane_cp;
@@ -255,61 +255,61 @@ __device__ __attribute__((always_inline)) void tile_pass(const TileCtx& x, int c
       float rt[P2 + 1];
 #pragma unroll
       for (int q = 0; q <= P2; ++q) rt[q] = sgm == 0 ? x.rtab[0][q] : x.rtab[1][q];
-    for (int l = 0; l < nr; ++l) {
-      const int s2 = __builtin_amdgcn_readlane(rs, l);
-      float bb[P2 + 1];
+      for (int l = 0; l < nr; ++l) {
+        const int s2 = __builtin_amdgcn_readlane(rs, l);
+        float bb[P2 + 1];
 #pragma unroll
-      for (int q = 0; q <= P2; ++q) bb[q] = readlane_f(rt[q], l);
-      if (s2 != cur) {
-        if (s2 - cur > P2) {
-          cur = s2;
+        for (int q = 0; q <= P2; ++q) bb[q] = readlane_f(rt[q], l);
+        if (s2 != cur) {
+          if (s2 - cur > P2) {
+            cur = s2;
 #pragma unroll
-          for (int q = 0; q <= P2; ++q) column(cur - P2 + q, V[q]);
-        } else {
-          while (cur < s2) {
+            for (int q = 0; q <= P2; ++q) column(cur - P2 + q, V[q]);
+          } else {
+            while (cur < s2) {
 #pragma unroll
-            for (int q = 0; q < P2; ++q)
+              for (int q = 0; q < P2; ++q)
 #pragma unroll
-              for (int h = 0; h < VEC; ++h)
+                for (int h = 0; h < VEC; ++h)
 #pragma unroll
-                for (int ch = 0; ch < NCH; ++ch) V[q][h][ch] = V[q + 1][h][ch];
-            ++cur;
-            column(cur, V[P2]);
+                  for (int ch = 0; ch < NCH; ++ch) V[q][h][ch] = V[q + 1][h][ch];
+              ++cur;
+              column(cur, V[P2]);
+            }
           }
         }
-      }
-      float val[VEC][NC];
+        float val[VEC][NC];
 #pragma unroll
-      for (int h = 0; h < VEC; ++h) {
-        float num[NCH];
+        for (int h = 0; h < VEC; ++h) {
+          float num[NCH];
 #pragma unroll
-        for (int ch = 0; ch < NCH; ++ch) {
-          float s = 0.f;
+          for (int ch = 0; ch < NCH; ++ch) {
+            float s = 0.f;
 #pragma unroll
-          for (int q = 0; q <= P2; ++q) s += bb[q] * V[q][h][ch];
-          num[ch] = s;
+            for (int q = 0; q <= P2; ++q) s += bb[q] * V[q][h][ch];
+            num[ch] = s;
+          }
+          const float inv = NURBS ? nurbs_recip(num[NC]) : 1.f;
+#pragma unroll
+          for (int c = 0; c < NC; ++c) val[h][c] = NURBS ? num[c] * inv : num[c];
         }
-        const float inv = NURBS ? nurbs_recip(num[NC]) : 1.f;
+        if constexpr (VEC == 2) {
+          // both samples: one 8-byte store per channel (4-byte aligned, as
+          // global stores allow); the tile's last odd sample alone: 4 bytes
+          if (x.activeb) {
 #pragma unroll
-        for (int c = 0; c < NC; ++c) val[h][c] = NURBS ? num[c] * inv : num[c];
-      }
-      if constexpr (VEC == 2) {
-        // both samples: one 8-byte store per channel (4-byte aligned, as
-        // global stores allow); the tile's last odd sample alone: 4 bytes
-        if (x.activeb) {
+            for (int c = 0; c < NC; ++c)
+              *reinterpret_cast<Global<f2u>*>(x.out + o + x.nsamp * c) = f2u{val[0][c], val[1][c]};
+          } else if (x.active) {
 #pragma unroll
-          for (int c = 0; c < NC; ++c)
-            *reinterpret_cast<Global<f2u>*>(x.out + o + x.nsamp * c) = f2u{val[0][c], val[1][c]};
+            for (int c = 0; c < NC; ++c) x.out[o + x.nsamp * c] = val[0][c];
+          }
         } else if (x.active) {
 #pragma unroll
           for (int c = 0; c < NC; ++c) x.out[o + x.nsamp * c] = val[0][c];
         }
-      } else if (x.active) {
-#pragma unroll
-        for (int c = 0; c < NC; ++c) x.out[o + x.nsamp * c] = val[0][c];
+        o += x.m1;
       }
-      o += x.m1;
-    }
     }
   }
   __syncthreads();  // G is rewritten by the next channel block / tile
@@ -370,7 +370,6 @@ __device__ __attribute__((always_inline)) void batch_tile(const BGrid& g, const 
   x.W = global_ptr(g.W);
   x.out = global_ptr(g.out);
   x.n1 = g.n1;
-  x.p3 = g.p3;
   const int width = min(tw, g.m1 - j1_0);  // samples j1 of the tile
   const int groups = zpack && VEC == 1 ? (width <= 16 ? 4 : width <= 32 ? 2 : 1) : 1;
   const int group_lanes = kLanes / groups;
