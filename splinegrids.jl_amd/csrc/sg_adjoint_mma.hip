@@ -45,6 +45,7 @@ constexpr int kBufs = 8;  // ebar units per unrolled group; kBufs - 1 are in fli
 
 struct AdjMmaArgs {
   const float* E;
+  const float* S;  // scale field [m3][m2][m1] applied to E as it loads (SCALE)
   float* part;
   const float* tab1;  // [m1][p1+1]
   const float* tab2;  // [m2][p2+1]
@@ -71,9 +72,12 @@ __device__ inline int j1_of(int g, int s) { return 16 * (s >> 2) + 4 * g + (s & 
 constexpr unsigned kOff = 0x80000000u;   // always past a plane (< 2 GiB)
 constexpr int kRsrcWord3 = 0x00020000;   // raw buffer, 32-bit elements (gfx9)
 
-template <int P3, int NB, bool VEC, int TIER>
+// SCALE: ebar times the field S, loaded at the same offsets into buffers of
+// its own; half as many units in flight keeps the registers of both
+template <int P3, int NB, bool VEC, int TIER, bool SCALE>
 __global__ __launch_bounds__(kThreads) void adj_mma(AdjMmaArgs a) {
-  static_assert(kBufs % NB == 0, "row block of a buffer position must be constant");
+  constexpr int KB = SCALE ? kBufs / 2 : kBufs;  // buffer positions
+  static_assert(KB % NB == 0, "row block of a buffer position must be constant");
   const int lane = threadIdx.x & (kLanes - 1);
   const int task = static_cast<int>(blockIdx.x) * kWaves +
                    __builtin_amdgcn_readfirstlane(threadIdx.x / kLanes);
@@ -170,9 +174,10 @@ __global__ __launch_bounds__(kThreads) void adj_mma(AdjMmaArgs a) {
   // read nothing: their descriptor has no records.
   int lc = 0, ljl = 0;
   const float* lplane = a.E + static_cast<long long>(j3a) * a.m12 + a.nsamp * a.c0;
-  auto load = [&](f4 (&e)[4], int b) {
+  const float* splane = SCALE ? a.S + static_cast<long long>(j3a) * a.m12 : nullptr;
+  auto load_plane = [&](const float* plane, f4 (&e)[4], int b) {
     const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
-        const_cast<float*>(lplane), 0, static_cast<int>(lc < a.nc ? plane_bytes : 0u),
+        const_cast<float*>(plane), 0, static_cast<int>(lc < a.nc ? plane_bytes : 0u),
         kRsrcWord3);
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -189,25 +194,32 @@ __global__ __launch_bounds__(kThreads) void adj_mma(AdjMmaArgs a) {
         }
       }
     }
+  };
+  auto load = [&](f4 (&e)[4], f4 (&d)[4], int b) {
+    load_plane(lplane, e, b);
+    if constexpr (SCALE) load_plane(splane, d, b);
     if (b == NB - 1) {  // the plane's last row block: advance the cursor
       lplane += a.m12;
+      if constexpr (SCALE) splane += a.m12;
       if (++ljl == J) {
         ljl = 0;
         ++lc;
         lplane = a.E + static_cast<long long>(j3a) * a.m12 + a.nsamp * (a.c0 + lc);
+        if constexpr (SCALE) splane = a.S + static_cast<long long>(j3a) * a.m12;
       }
     }
   };
 
-  // Units are taken in unrolled groups of kBufs with one register buffer per
-  // position: at position k the unit kBufs - 1 ahead is loaded into the
-  // buffer that position k - 1 has just used, so kBufs - 1 units are in
+  // Units are taken in unrolled groups of KB with one register buffer per
+  // position: at position k the unit KB - 1 ahead is loaded into the
+  // buffer that position k - 1 has just used, so KB - 1 units are in
   // flight and no buffer is ever copied (a copy of a register whose load is
-  // in flight waits for that load).  kBufs % NB == 0 keeps the row block of
+  // in flight waits for that load).  KB % NB == 0 keeps the row block of
   // position k, k % NB, a constant.
-  f4 buf[kBufs][4];
+  f4 buf[KB][4];
+  f4 sbuf[SCALE ? KB : 1][4];  // the scale of each buffered unit (SCALE)
 #pragma unroll
-  for (int k = 0; k < kBufs - 1; ++k) load(buf[k], k % NB);
+  for (int k = 0; k < KB - 1; ++k) load(buf[k], sbuf[SCALE ? k : 0], k % NB);
 
   f4 R[P3 + 1];  // ring of the current channel: slot q holds k3 = cur3 - P3 + q
 #pragma unroll
@@ -234,13 +246,17 @@ __global__ __launch_bounds__(kThreads) void adj_mma(AdjMmaArgs a) {
 
   // compute cursor: channel and sample of the unit being multiplied
   int cc = 0, cjl = 0;
-  for (int u0 = 0; u0 < U; u0 += kBufs) {
+  for (int u0 = 0; u0 < U; u0 += KB) {
 #pragma unroll
-    for (int k = 0; k < kBufs; ++k) {
+    for (int k = 0; k < KB; ++k) {
       const int u = u0 + k;
-      load(buf[(k + kBufs - 1) % kBufs], (k + kBufs - 1) % NB);
+      load(buf[(k + KB - 1) % KB], sbuf[SCALE ? (k + KB - 1) % KB : 0], (k + KB - 1) % NB);
       if (u < U) {
         const int b = k % NB;
+        if constexpr (SCALE) {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) buf[k][q] *= sbuf[k][q];
+        }
         f4 d1 = {0.f, 0.f, 0.f, 0.f};
         if constexpr (TIER == SG_TIER_NATIVE) {
 #pragma unroll
@@ -319,38 +335,40 @@ __global__ __launch_bounds__(kThreads) void adj_mma(AdjMmaArgs a) {
 using AdjMmaKernel = void (*)(AdjMmaArgs);
 
 template <int P3, int NB, bool VEC>
-AdjMmaKernel pick_tier(int tier) {
+AdjMmaKernel pick_tier(int tier, bool scale) {
+  if (scale)  // the scaled form: native tier only
+    return tier == SG_TIER_NATIVE ? adj_mma<P3, NB, VEC, SG_TIER_NATIVE, true> : nullptr;
   switch (tier) {
-    case SG_TIER_NATIVE: return adj_mma<P3, NB, VEC, SG_TIER_NATIVE>;
-    case SG_TIER_SPLIT_F16: return adj_mma<P3, NB, VEC, SG_TIER_SPLIT_F16>;
-    case SG_TIER_BF16: return adj_mma<P3, NB, VEC, SG_TIER_BF16>;
+    case SG_TIER_NATIVE: return adj_mma<P3, NB, VEC, SG_TIER_NATIVE, false>;
+    case SG_TIER_SPLIT_F16: return adj_mma<P3, NB, VEC, SG_TIER_SPLIT_F16, false>;
+    case SG_TIER_BF16: return adj_mma<P3, NB, VEC, SG_TIER_BF16, false>;
   }
   return nullptr;
 }
 
 template <int P3, int NB>
-AdjMmaKernel pick_vec(bool vec, int tier) {
-  return vec ? pick_tier<P3, NB, true>(tier) : pick_tier<P3, NB, false>(tier);
+AdjMmaKernel pick_vec(bool vec, int tier, bool scale) {
+  return vec ? pick_tier<P3, NB, true>(tier, scale) : pick_tier<P3, NB, false>(tier, scale);
 }
 
 template <int P3>
-AdjMmaKernel pick_nb(int nb, bool vec, int tier) {
+AdjMmaKernel pick_nb(int nb, bool vec, int tier, bool scale) {
   switch (nb) {
-    case 1: return pick_vec<P3, 1>(vec, tier);
-    case 2: return pick_vec<P3, 2>(vec, tier);
-    case 4: return pick_vec<P3, 4>(vec, tier);
+    case 1: return pick_vec<P3, 1>(vec, tier, scale);
+    case 2: return pick_vec<P3, 2>(vec, tier, scale);
+    case 4: return pick_vec<P3, 4>(vec, tier, scale);
   }
   return nullptr;
 }
 
-AdjMmaKernel pick_adj_mma(int p3, int nb, bool vec, int tier) {
+AdjMmaKernel pick_adj_mma(int p3, int nb, bool vec, int tier, bool scale = false) {
   switch (p3) {
-    case 0: return pick_nb<0>(nb, vec, tier);
-    case 1: return pick_nb<1>(nb, vec, tier);
-    case 2: return pick_nb<2>(nb, vec, tier);
-    case 3: return pick_nb<3>(nb, vec, tier);
-    case 4: return pick_nb<4>(nb, vec, tier);
-    case 5: return pick_nb<5>(nb, vec, tier);
+    case 0: return pick_nb<0>(nb, vec, tier, scale);
+    case 1: return pick_nb<1>(nb, vec, tier, scale);
+    case 2: return pick_nb<2>(nb, vec, tier, scale);
+    case 3: return pick_nb<3>(nb, vec, tier, scale);
+    case 4: return pick_nb<4>(nb, vec, tier, scale);
+    case 5: return pick_nb<5>(nb, vec, tier, scale);
   }
   return nullptr;
 }
@@ -388,12 +406,13 @@ size_t adjoint_mma_work_bytes(const Plan& pl) {
 }
 
 hipError_t launch_adjoint_mma(const Plan& pl, int tier, const int32_t* orders, const float* E,
-                              float* Pbar, int accumulate, hipStream_t s) {
+                              float* Pbar, int accumulate, hipStream_t s, const float* scale) {
   const Tiling& tl = pl.adjm;
   const DimDev& d1 = pl.dims[0];
   const DimDev& d2 = pl.dims[1];
   AdjMmaArgs a{};
   a.E = E;
+  a.S = scale;
   a.part = static_cast<float*>(pl.work);
   a.tab1 = d1.ktab32 + static_cast<size_t>(orders[0]) * d1.m * (d1.p + 1);
   a.tab2 = d2.ktab32 + static_cast<size_t>(orders[1]) * d2.m * (d2.p + 1);
@@ -423,9 +442,10 @@ hipError_t launch_adjoint_mma(const Plan& pl, int tier, const int32_t* orders, c
   a.m12 = static_cast<long long>(d1.m) * d2.m;
   a.nsamp = a.m12 * a.m3;
   // 16-byte loads need 16-byte rows: m1 % 4 == 0 and an aligned base
-  const bool vec = (d1.m % 4 == 0) && (reinterpret_cast<uintptr_t>(E) % 16 == 0);
+  const bool vec = (d1.m % 4 == 0) && (reinterpret_cast<uintptr_t>(E) % 16 == 0) &&
+                   (reinterpret_cast<uintptr_t>(scale) % 16 == 0);
   const dim3 grid(static_cast<unsigned>((a.ntasks + kWaves - 1) / kWaves));
-  AdjMmaKernel k = pick_adj_mma(plan_p3(pl), tl.chunk_rows() / 16, vec, tier);
+  AdjMmaKernel k = pick_adj_mma(plan_p3(pl), tl.chunk_rows() / 16, vec, tier, scale != nullptr);
   if (!k) return hipErrorInvalidValue;
   a.c0 = 0;
   a.nc = pl.nout;

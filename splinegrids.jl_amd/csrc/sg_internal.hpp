@@ -254,7 +254,11 @@ hipError_t launch_adjoint_gather(const Plan& plan, const Tiling& tl,
 bool adjoint_mma_shape_ok(const Plan& plan);
 int adjoint_mma_resident_waves(const Plan& plan);  // on the current device
 size_t adjoint_mma_work_bytes(const Plan& plan);
+// scale (native tier only): when not null, ebar is multiplied sample by
+// sample by the field scale[j3][j2][j1], shared by the channels, as it is
+// loaded (the NURBS adjoint's 1 / w(x))
 hipError_t launch_adjoint_mma(const Plan& plan, int tier, const int32_t* orders, const float* ebar,
-                              float* Pbar, int accumulate, hipStream_t s);
+                              float* Pbar, int accumulate, hipStream_t s,
+                              const float* scale = nullptr);
 
 }  // namespace sg

@@ -4,9 +4,11 @@
 // src/adjoint.jl:57).
 //
 // Adjoint: with the weights fixed R is linear in P, R = B (w P) / w(x), so
-// Pbar_k = w_k [B^T (ebar / w(x))]_k: ebar / w(x) in one streaming pass (the
-// weight field in registers, as the B-spline forward computes it), the
-// B-spline adjoint, a product.
+// Pbar_k = w_k [B^T (ebar / w(x))]_k.  FP32 grids on the matrix-core adjoint:
+// 1 / w(x) in one streaming pass (the weight field in registers, as the
+// B-spline forward computes it), then the B-spline adjoint multiplying ebar
+// by it as it loads ebar, then a product.  Otherwise ebar / w(x) is written
+// by that pass and the B-spline adjoint reads it.
 //
 // Two passes.  (1) The homogeneous control points (w P_c, w) go through the
 // B-spline forward (launch_forward on a view of the plan without weights and
@@ -387,7 +389,7 @@ void* combine_kernel_k(int k) {
 // ebar / w(x) for every channel in one pass: the forward's tiles over the
 // weights as a one-channel B-spline (stage 1 window of w, the register
 // window over k2), and each row's w(x) divides the row of ebar of every
-// channel as it is read.
+// channel as it is read; with E null, the pass writes 1 / w(x) instead.
 template <typename T>
 struct DivArgs {
   BandSrc<T> src;  // P = the weights, one channel
@@ -464,7 +466,10 @@ __global__ __launch_bounds__(kThreads) void divide_by_weight_field(
 #pragma unroll
     for (int q = 0; q <= P2; ++q) w += bb[q] * V[q];
     const T inv = T(1) / w;
-    for (int c = 0; c < a.nout; ++c) a.out[o + a.nsamp * c] = a.E[o + a.nsamp * c] * inv;
+    if (a.E == nullptr)
+      a.out[o] = inv;
+    else
+      for (int c = 0; c < a.nout; ++c) a.out[o + a.nsamp * c] = a.E[o + a.nsamp * c] * inv;
   }
 }
 
@@ -492,8 +497,8 @@ __global__ void divide_by_field(const T* E, const T* D, T* out, long long nsamp,
     out[i + nsamp * c] = static_cast<T>(static_cast<double>(E[i + nsamp * c]) * inv);
 }
 
-// The one-pass division: streaming grids (2-D, 3-D) whose weight window
-// fits; false otherwise.
+// The one-pass division (or, with E null, 1 / w(x) alone): streaming grids
+// (2-D, 3-D) whose weight window fits; false otherwise.
 template <typename T>
 bool divide_streaming(const Plan& pl, const T* W, const T* E, T* out, hipStream_t s,
                       hipError_t& e) {
@@ -567,6 +572,19 @@ hipError_t rational_adjoint(const Plan& pl, const T* W, const T* E, T* Pbar, voi
   const int dtype = sizeof(T) == 4 ? SG_F32 : SG_F64;
   const int32_t zero[SG_MAX_NIN] = {0, 0, 0, 0, 0, 0, 0, 0};
   hipError_t e = hipSuccess;
+  Plan bp = pl;  // a view: the B-spline adjoint of the same dimensions
+  bp.nurbs = 0;
+  if constexpr (sizeof(T) == 4) {
+    // 1 / w(x) into D, then the matrix-core adjoint scales ebar as it loads
+    if (nsamp > 0 && pl.adjm.n_tiles1 > 0 && divide_streaming<T>(pl, W, nullptr, D, s, e)) {
+      if (e != hipSuccess) return e;
+      e = launch_adjoint_mma(bp, SG_TIER_NATIVE, zero, E, Q, 0, s, D);
+      if (e != hipSuccess) return e;
+      hipLaunchKernelGGL(weight_products<T>, dim3(static_cast<unsigned>((ncp + 255) / 256)),
+                         dim3(256), 0, s, Q, W, Pbar, ncp, pl.nout, accumulate);
+      return hipGetLastError();
+    }
+  }
   if (nsamp > 0 && !divide_streaming(pl, W, E, Et, s, e)) {
     Plan wp = pl;  // a view: the weights as a one-channel B-spline
     wp.nurbs = 0;
@@ -579,8 +597,6 @@ hipError_t rational_adjoint(const Plan& pl, const T* W, const T* E, T* Pbar, voi
     e = hipGetLastError();
   }
   if (e != hipSuccess) return e;
-  Plan bp = pl;  // a view: the B-spline adjoint of the same dimensions
-  bp.nurbs = 0;
   e = launch_adjoint(bp, zero, Et, Q, dtype, 0, s);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(weight_products<T>, dim3(static_cast<unsigned>((ncp + 255) / 256)),
