@@ -73,10 +73,19 @@ constexpr unsigned kOff = 0x80000000u;   // always past a plane (< 2 GiB)
 constexpr int kRsrcWord3 = 0x00020000;   // raw buffer, 32-bit elements (gfx9)
 
 // SCALE: ebar times the field S, loaded at the same offsets into buffers of
-// its own; half as many units in flight keeps the registers of both
+// its own; half as many units in flight keeps the registers of both.  With
+// p3 <= kScaleMaxP3 the units go sample by sample, the channels (at most
+// kScaleChannels per launch) of a sample together, each channel with a ring
+// of its own, so a unit of S is read again by the next channel while it is
+// still in L2 rather than once per channel pass over the task.
+constexpr int kScaleMaxP3 = 3;
+constexpr int kScaleChannels = 3;
+
 template <int P3, int NB, bool VEC, int TIER, bool SCALE>
 __global__ __launch_bounds__(kThreads) void adj_mma(AdjMmaArgs a) {
   constexpr int KB = SCALE ? kBufs / 2 : kBufs;  // buffer positions
+  constexpr bool CM = SCALE && P3 <= kScaleMaxP3;  // channels in the middle
+  constexpr int NCM = CM ? kScaleChannels : 1;
   static_assert(KB % NB == 0, "row block of a buffer position must be constant");
   const int lane = threadIdx.x & (kLanes - 1);
   const int task = static_cast<int>(blockIdx.x) * kWaves +
@@ -176,9 +185,10 @@ __global__ __launch_bounds__(kThreads) void adj_mma(AdjMmaArgs a) {
   const float* lplane = a.E + static_cast<long long>(j3a) * a.m12 + a.nsamp * a.c0;
   const float* splane = SCALE ? a.S + static_cast<long long>(j3a) * a.m12 : nullptr;
   auto load_plane = [&](const float* plane, f4 (&e)[4], int b) {
+    // units past the task's last read nothing: their descriptor has no records
+    const bool live = CM ? ljl < J : lc < a.nc;
     const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
-        const_cast<float*>(plane), 0, static_cast<int>(lc < a.nc ? plane_bytes : 0u),
-        kRsrcWord3);
+        const_cast<float*>(plane), 0, static_cast<int>(live ? plane_bytes : 0u), kRsrcWord3);
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int j1 = j1_0 + 16 * q + 4 * g;
@@ -199,13 +209,22 @@ __global__ __launch_bounds__(kThreads) void adj_mma(AdjMmaArgs a) {
     load_plane(lplane, e, b);
     if constexpr (SCALE) load_plane(splane, d, b);
     if (b == NB - 1) {  // the plane's last row block: advance the cursor
-      lplane += a.m12;
-      if constexpr (SCALE) splane += a.m12;
-      if (++ljl == J) {
-        ljl = 0;
-        ++lc;
-        lplane = a.E + static_cast<long long>(j3a) * a.m12 + a.nsamp * (a.c0 + lc);
-        if constexpr (SCALE) splane = a.S + static_cast<long long>(j3a) * a.m12;
+      if constexpr (CM) {  // units (sample, channel, row block)
+        if (++lc == a.nc) {
+          lc = 0;
+          ++ljl;
+          splane += a.m12;
+        }
+        lplane = a.E + static_cast<long long>(j3a + ljl) * a.m12 + a.nsamp * (a.c0 + lc);
+      } else {  // units (channel, sample, row block)
+        lplane += a.m12;
+        if constexpr (SCALE) splane += a.m12;
+        if (++ljl == J) {
+          ljl = 0;
+          ++lc;
+          lplane = a.E + static_cast<long long>(j3a) * a.m12 + a.nsamp * (a.c0 + lc);
+          if constexpr (SCALE) splane = a.S + static_cast<long long>(j3a) * a.m12;
+        }
       }
     }
   };
@@ -224,6 +243,15 @@ __global__ __launch_bounds__(kThreads) void adj_mma(AdjMmaArgs a) {
   f4 R[P3 + 1];  // ring of the current channel: slot q holds k3 = cur3 - P3 + q
 #pragma unroll
   for (int q = 0; q <= P3; ++q) R[q] = f4{0.f, 0.f, 0.f, 0.f};
+  // CM: a ring per channel, and the B3 row of the current sample
+  f4 RC[NCM][CM ? P3 + 1 : 1];
+  float b3c[P3 + 1];
+  if constexpr (CM) {
+#pragma unroll
+    for (int ch = 0; ch < NCM; ++ch)
+#pragma unroll
+      for (int q = 0; q <= P3; ++q) RC[ch][q] = f4{0.f, 0.f, 0.f, 0.f};
+  }
   int cur3 = a.span3[j3a];
   f4 acc = {0.f, 0.f, 0.f, 0.f};  // T2 of the current sample j3
   // B3 row of the next sample, loaded one sample ahead (before that sample's
@@ -291,7 +319,53 @@ __global__ __launch_bounds__(kThreads) void adj_mma(AdjMmaArgs a) {
 #pragma unroll
         for (int r = 0; r < 4; ++r)
           acc = __builtin_amdgcn_mfma_f32_16x16x4f32(b2[b][r], d1[r], acc, 0, 0, 0);
-        if (b == NB - 1) {
+        if (CM && b == NB - 1) {
+          // T2 of sample j3, channel c is complete: at the sample's first
+          // channel every ring moves to the sample's band; then channel c's
+          // ring takes it
+          const int c = cc, jl = cjl;
+          const int j3 = j3a + jl;
+          if (++cc == a.nc) {
+            cc = 0;
+            ++cjl;
+          }
+          if (c == 0) {
+#pragma unroll
+            for (int q = 0; q <= P3; ++q) b3c[q] = b3n[q];
+            const int j3n = jl + 1 < J ? j3 + 1 : j3;
+#pragma unroll
+            for (int q = 0; q <= P3; ++q)
+              b3n[q] = a.tab3[static_cast<size_t>(j3n) * (P3 + 1) + q];
+            const int s3 = a.span3[j3];
+            while (cur3 < s3) {
+#pragma unroll
+              for (int ch = 0; ch < NCM; ++ch) {
+                if (ch < a.nc) flush(RC[ch][0], ch, cur3 - P3);
+#pragma unroll
+                for (int q = 0; q < P3; ++q) RC[ch][q] = RC[ch][q + 1];
+                RC[ch][P3] = f4{0.f, 0.f, 0.f, 0.f};
+              }
+              ++cur3;
+            }
+          }
+#pragma unroll
+          for (int ch = 0; ch < NCM; ++ch)
+            if (ch == c) {  // c is uniform: a scalar branch, no register indexing
+#pragma unroll
+              for (int q = 0; q <= P3; ++q)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) RC[ch][q][r] += b3c[q] * acc[r];
+            }
+          acc = f4{0.f, 0.f, 0.f, 0.f};
+          if (jl == J - 1 && c == a.nc - 1) {  // the task's last unit: every ring is final
+#pragma unroll
+            for (int ch = 0; ch < NCM; ++ch)
+              if (ch < a.nc) {
+#pragma unroll
+                for (int q = 0; q <= P3; ++q) flush(RC[ch][q], ch, cur3 - P3 + q);
+              }
+          }
+        } else if (!CM && b == NB - 1) {
           // T2 of sample j3 is complete: weight it into the ring
           const int c = cc, jl = cjl;
           const int j3 = j3a + jl;
@@ -447,10 +521,15 @@ hipError_t launch_adjoint_mma(const Plan& pl, int tier, const int32_t* orders, c
   const dim3 grid(static_cast<unsigned>((a.ntasks + kWaves - 1) / kWaves));
   AdjMmaKernel k = pick_adj_mma(plan_p3(pl), tl.chunk_rows() / 16, vec, tier, scale != nullptr);
   if (!k) return hipErrorInvalidValue;
-  a.c0 = 0;
-  a.nc = pl.nout;
-  hipLaunchKernelGGL(k, grid, dim3(kThreads), 0, s, a);
-  hipError_t e = hipGetLastError();
+  // the scaled form with channels in the middle takes kScaleChannels per launch
+  const int per = scale != nullptr && plan_p3(pl) <= kScaleMaxP3 ? kScaleChannels : pl.nout;
+  hipError_t e = hipSuccess;
+  for (int c0 = 0; c0 < pl.nout && e == hipSuccess; c0 += per) {
+    a.c0 = c0;
+    a.nc = std::min(per, pl.nout - c0);
+    hipLaunchKernelGGL(k, grid, dim3(kThreads), 0, s, a);
+    e = hipGetLastError();
+  }
   if (e != hipSuccess) return e;
   return launch_adjoint_gather(pl, tl, pl.work, Pbar, SG_F32, accumulate, s);
 }
