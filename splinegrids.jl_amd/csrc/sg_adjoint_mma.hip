@@ -74,19 +74,20 @@ constexpr int kRsrcWord3 = 0x00020000;   // raw buffer, 32-bit elements (gfx9)
 
 // SCALE: ebar times the field S, loaded at the same offsets into buffers of
 // its own; half as many units in flight keeps the registers of both.
-// CM (the native tier and the scaled form, p3 <= kCmMaxP3): the units go
+// CM (every tier and the scaled form, p3 <= kCmMaxP3): the units go
 // sample by sample, the channels (at most kCmChannels per launch) of a
 // sample together, each channel with a ring of its own: B3 rows and ring
 // moves once per sample rather than once per channel and sample, and a unit
 // of S is read again by the next channel while it is still in L2 rather
-// than once per channel pass over the task (C4: 95.3 -> 90.6 us).
+// than once per channel pass over the task (C4: 95.3 -> 90.6 us; the split-FP16
+// and BF16 tiers 100 -> 98 and 95 us).
 constexpr int kCmMaxP3 = 3;
 constexpr int kCmChannels = 3;
 
 template <int P3, int NB, bool VEC, int TIER, bool SCALE>
 __global__ __launch_bounds__(kThreads) void adj_mma(AdjMmaArgs a) {
   constexpr int KB = SCALE ? kBufs / 2 : kBufs;  // buffer positions
-  constexpr bool CM = (SCALE || TIER == SG_TIER_NATIVE) && P3 <= kCmMaxP3;
+  constexpr bool CM = P3 <= kCmMaxP3;
   constexpr int NCM = CM ? kCmChannels : 1;
   static_assert(KB % NB == 0, "row block of a buffer position must be constant");
   const int lane = threadIdx.x & (kLanes - 1);
@@ -523,10 +524,8 @@ hipError_t launch_adjoint_mma(const Plan& pl, int tier, const int32_t* orders, c
   const dim3 grid(static_cast<unsigned>((a.ntasks + kWaves - 1) / kWaves));
   AdjMmaKernel k = pick_adj_mma(plan_p3(pl), tl.chunk_rows() / 16, vec, tier, scale != nullptr);
   if (!k) return hipErrorInvalidValue;
-  // channels in the middle (the native tier and the scaled form): kCmChannels per launch
-  const int per = (scale != nullptr || tier == SG_TIER_NATIVE) && plan_p3(pl) <= kCmMaxP3
-                      ? kCmChannels
-                      : pl.nout;
+  // channels in the middle: kCmChannels per launch
+  const int per = plan_p3(pl) <= kCmMaxP3 ? kCmChannels : pl.nout;
   hipError_t e = hipSuccess;
   for (int c0 = 0; c0 < pl.nout && e == hipSuccess; c0 += per) {
     a.c0 = c0;
